@@ -1,0 +1,305 @@
+"""NumPy restatement of trg/ATRG_3d_new.py's pure-tensor path (test infrastructure only).
+
+Tensor layout: T_{TXYtxy} = U_{TXYi} s_i VH_{itxy} (an ``ATRG_Tensor``).
+One step (trg/ATRG_3d_new.py:441-482): leg transposition, B = s.VH and
+C = U.s, leg swapping by randomized SVD, squeezer, coarse graining, restore.
+
+Differences from the reference, all documented:
+  * the rSVD sketch comes from the Philox stream (oracle/philox.py) rather
+    than cuRAND ``RandomState(12345)`` (:90-91);
+  * the rSVD parameters default to the reference's (k = 3 chi_k, p = 0,
+    q = Dcut, :193) and can be switched to the workload's (k = chi_k,
+    p = chi, q = 2) with ``set_rsvd_params``;
+  * ``DGENERACY_EPS`` defaults to 0.0 (the reference requires the env var,
+    :15); ``OUTPUT_DIR`` file output (:197-212, :736-766) is opt-in.
+"""
+import itertools
+import math
+import os
+
+import numpy as np
+
+from .philox import standard_normal
+from .tensor_class import ATRG_Tensor as Tensor
+from .truncated_svd import svd, truncate
+
+degeneracy_eps = float(os.environ.get("DGENERACY_EPS", "0.0"))
+truncate_eps = 1e-10
+
+# rSVD parameters of the leg swap: rank multiplier, oversampling, QR iterations
+# (None = Dcut as in the reference), seed.
+_RSVD = {"k_factor": 3, "p": 0, "q": None, "seed": 12345}
+
+
+def set_rsvd_params(k_factor=3, p=0, q=None, seed=12345):
+    """Select the leg-swap rSVD parameters (reference: 3, 0, Dcut, 12345)."""
+    _RSVD.update(k_factor=k_factor, p=p, q=q, seed=seed)
+
+
+def get_rsvd_params():
+    return dict(_RSVD)
+
+
+def _conj(A):
+    return np.conj(A) if np.iscomplexobj(A) else A
+
+
+def leg_transposition(A, do_what="transpose", direction="t"):
+    """trg/ATRG_3d_new.py:18-65."""
+    if direction in ("t", "T"):
+        return A
+    fwd = do_what == "transpose"
+    if direction in ("x", "X"):
+        ua, va = ((1, 2, 0, 3), (0, 2, 3, 1)) if fwd else ((2, 0, 1, 3), (0, 3, 1, 2))
+    else:
+        ua, va = ((2, 0, 1, 3), (0, 3, 1, 2)) if fwd else ((1, 2, 0, 3), (0, 2, 3, 1))
+    A.U = np.ascontiguousarray(np.transpose(A.U, ua))
+    A.VH = np.ascontiguousarray(np.transpose(A.VH, va))
+    return A
+
+
+def _qr(Y):
+    q, _ = np.linalg.qr(Y)
+    return q
+
+
+def rsvd_for_ATRG_leg_swapping(B, C, chi_k, p=32, q=64, eps=1e-10, seed=12345):
+    """trg/ATRG_3d_new.py:67-140: matrix-free rSVD of
+    M_{(i,g),(j,b)} = sum_t B_{b,i,t} C_{t,j,g}.  Returns UM_{i,g,k}, sM_k, VMH_{k,j,b}.
+    """
+    nb, ni, nt = B.shape
+    _, nj, ng = C.shape
+    l = chi_k + p
+    Y = standard_normal((nj, nb, l), seed)
+    # Y'_{i g l} = sum B_{b i t} C_{t j g} Y_{j b l}
+    W = np.tensordot(C, Y, axes=([1], [0]))                 # (t,g,b,l)
+    Y = np.tensordot(B, W, axes=([0, 2], [2, 0]))           # (i,g,l)
+    Q = _qr(Y.reshape(ni * ng, l))
+    Bc, Cc = _conj(B), _conj(C)
+    for _ in range(q):
+        ls = Q.shape[1]
+        Qt = Q.reshape(ni, ng, ls)
+        # (M^dag Q)_{j b l} = sum B*_{b i t} C*_{t j g} Q_{i g l}
+        W = np.tensordot(Bc, Qt, axes=([1], [0]))           # (b,t,g,l)
+        Y = np.tensordot(Cc, W, axes=([0, 2], [1, 2]))      # (j,b,l)
+        Q = _qr(Y.reshape(nj * nb, ls))
+        ls = Q.shape[1]
+        Qt = Q.reshape(nj, nb, ls)
+        W = np.tensordot(C, Qt, axes=([1], [0]))            # (t,g,b,l)
+        Y = np.tensordot(B, W, axes=([0, 2], [2, 0]))       # (i,g,l)
+        Q = _qr(Y.reshape(ni * ng, ls))
+    ls = Q.shape[1]
+    Qt = Q.reshape(ni, ng, ls)
+    # X_{l j b} = sum Q*_{i g l} B_{b i t} C_{t j g}
+    W = np.tensordot(_conj(Qt), B, axes=([0], [1]))         # (g,l,b,t)
+    X = np.tensordot(W, C, axes=([0, 3], [2, 0]))           # (l,b,j)
+    X = np.transpose(X, (0, 2, 1)).reshape(ls, nj * nb)
+    UX, sM, VMH = np.linalg.svd(X, full_matrices=False)
+    r = len(sM)
+    VMH = VMH.reshape(r, nj, nb)
+    UM = (Q @ UX).reshape(ni, ng, r)
+    s0 = sM / np.max(sM)
+    k = min(int(np.count_nonzero(s0 > eps)), chi_k)
+    return UM[:, :, :k], sM[:k], VMH[:k]
+
+
+def leg_swapping(B, C, Dcut, ispure=True):
+    """trg/ATRG_3d_new.py:142-235.  B_{i,t1,x1,y1}, C_{T0,X0,Y0,j} ->
+    UM_{i,X0,Y0,k}, sM_k, VMH_{k,j,x1,y1}."""
+    ni, nt1, nx1, ny1 = B.shape
+    nT0, nX0, nY0, nj = C.shape
+    Bm = np.transpose(B, (2, 3, 0, 1)).reshape(nx1 * ny1, ni * nt1)
+    UB, sB, VBH = np.linalg.svd(Bm, full_matrices=False)
+    beta = len(sB)
+    UB = UB.reshape(nx1, ny1, beta)
+    Bt = sB[:, None, None] * VBH.reshape(beta, ni, nt1)
+    Cm = np.transpose(C, (0, 3, 1, 2)).reshape(nT0 * nj, nX0 * nY0)
+    UC, sC, VCH = np.linalg.svd(Cm, full_matrices=False)
+    gamma = len(sC)
+    VCH = VCH.reshape(gamma, nX0, nY0)
+    Ct = UC.reshape(nT0, nj, gamma) * sC[None, None, :]
+    chi_k = min(Dcut, nj * beta, ni * gamma)
+    q = Dcut if _RSVD["q"] is None else _RSVD["q"]
+    UM, sM, VMH = rsvd_for_ATRG_leg_swapping(Bt, Ct, chi_k=_RSVD["k_factor"] * chi_k,
+                                             p=_RSVD["p"], q=q, eps=truncate_eps, seed=_RSVD["seed"])
+    chi, _ = truncate(sM, k=chi_k, degeneracy_eps=degeneracy_eps, truncate_eps=truncate_eps)
+    UM, sM, VMH = UM[:, :, :chi], sM[:chi], VMH[:chi]
+    UM = np.tensordot(UM, VCH, axes=([1], [0]))             # (i,k,X,Y)
+    UM = np.transpose(UM, (0, 2, 3, 1))
+    VMH = np.tensordot(VMH, UB, axes=([2], [2]))            # (k,j,x,y)
+    return UM, sM, VMH
+
+
+def squeezer_environments(A, UM, sM, VMH, D):
+    """The four environments of trg/ATRG_3d_new.py:262-265 as matrices."""
+    nX1, nY1 = A.shape[1], A.shape[2]
+    nX0, nY0 = UM.shape[1], UM.shape[2]
+    nx1, ny1 = VMH.shape[2], VMH.shape[3]
+    nx0, ny0 = D.shape[2], D.shape[3]
+    Ac, UMc, VMHc, Dc = _conj(A), _conj(UM), _conj(VMH), _conj(D)
+    UMs = UM * sM[None, None, None, :]
+    # XdagX[i,j,k,l] = sum A*[a,i,b,c] A[a,k,b,f] UM*[c,j,d,e] s[e] UM[f,l,d,e]
+    P = np.tensordot(Ac, A, axes=([0, 2], [0, 2]))          # (i,c,k,f)
+    Q = np.tensordot(UMc, UMs, axes=([2, 3], [2, 3]))       # (c,j,f,l)
+    XdX = np.tensordot(P, Q, axes=([1, 3], [0, 2]))          # (i,k,j,l)
+    XdX = np.transpose(XdX, (0, 2, 1, 3)).reshape(nX1 * nX0, nX1 * nX0)
+    # YdagY[i,j,k,l] = sum A*[a,b,i,c] A[a,b,k,f] UM*[c,d,j,e] s[e] UM[f,d,l,e]
+    P = np.tensordot(Ac, A, axes=([0, 1], [0, 1]))          # (i,c,k,f)
+    Q = np.tensordot(UMc, UMs, axes=([1, 3], [1, 3]))       # (c,j,f,l)
+    YdY = np.tensordot(P, Q, axes=([1, 3], [0, 2]))
+    YdY = np.transpose(YdY, (0, 2, 1, 3)).reshape(nY1 * nY0, nY1 * nY0)
+    VMHs = VMH * sM[:, None, None, None]
+    # xxdag[i,j,k,l] = sum D[b,e,j,f] D*[d,e,l,f] VMH[a,b,i,c] s[a] VMH*[a,d,k,c]
+    P = np.tensordot(D, Dc, axes=([1, 3], [1, 3]))          # (b,j,d,l)
+    Q = np.tensordot(VMHs, VMHc, axes=([0, 3], [0, 3]))     # (b,i,d,k)
+    xxd = np.tensordot(Q, P, axes=([0, 2], [0, 2]))          # (i,k,j,l)
+    xxd = np.transpose(xxd, (0, 2, 1, 3)).reshape(nx1 * nx0, nx1 * nx0)
+    # yydag[i,j,k,l] = sum D[b,e,f,j] D*[d,e,f,l] VMH[a,b,c,i] s[a] VMH*[a,d,c,k]
+    P = np.tensordot(D, Dc, axes=([1, 2], [1, 2]))          # (b,j,d,l)
+    Q = np.tensordot(VMHs, VMHc, axes=([0, 2], [0, 2]))     # (b,i,d,k)
+    yyd = np.tensordot(Q, P, axes=([0, 2], [0, 2]))
+    yyd = np.transpose(yyd, (0, 2, 1, 3)).reshape(ny1 * ny0, ny1 * ny0)
+    return XdX, YdY, xxd, yyd
+
+
+def _oblique_pair(Rbig, Rsmall, Dcut, shape_big, shape_small):
+    """:278-292 -- u,s,vh = svd(RX@Rx, k) ; PX = Rx v s^-1/2 ; Px = s^-1/2 u^dag RX."""
+    k1 = min(Rbig.shape[0], Rsmall.shape[1], Dcut)
+    u, s, vh = svd(Rbig @ Rsmall, shape=[[0], [1]], k=k1,
+                   degeneracy_eps=degeneracy_eps, truncate_eps=truncate_eps)
+    uh = np.conj(u.T)
+    v = np.conj(vh.T)
+    sis = np.sqrt(1.0 / s)
+    PX = (Rsmall @ v) * sis[None, :]
+    Px = sis[:, None] * (uh @ Rbig)
+    chi = min(len(s), Dcut)
+    PX = PX[:, :chi].reshape(shape_big + (chi,))
+    Px = Px[:chi, :].reshape((chi,) + shape_small)
+    return Px, PX
+
+
+def squeezer(A, UM, sM, VMH, D, Dcut):
+    """trg/ATRG_3d_new.py:246-317 -> Px_{x,x1,x0}, PX_{X1,X0,X}, Py, PY."""
+    XdX, YdY, xxd, yyd = squeezer_environments(A, UM, sM, VMH, D)
+    _, sX, vXh = np.linalg.svd(XdX, full_matrices=False)
+    _, sY, vYh = np.linalg.svd(YdY, full_matrices=False)
+    ux, sx, _ = np.linalg.svd(xxd, full_matrices=False)
+    uy, sy, _ = np.linalg.svd(yyd, full_matrices=False)
+    RX = np.sqrt(sX)[:, None] * vXh
+    Rx = ux * np.sqrt(sx)[None, :]
+    Px, PX = _oblique_pair(RX, Rx, Dcut, (A.shape[1], UM.shape[1]), (VMH.shape[2], D.shape[2]))
+    RY = np.sqrt(sY)[:, None] * vYh
+    Ry = uy * np.sqrt(sy)[None, :]
+    Py, PY = _oblique_pair(RY, Ry, Dcut, (A.shape[2], UM.shape[2]), (VMH.shape[3], D.shape[3]))
+    return Px, PX, Py, PY
+
+
+def leg_slicing(leg_size, slicing):
+    """trg/ATRG_3d_new.py:319-355 (slice list of a leg).  A zero-size leg
+    raises ValueError instead of the reference's sys.exit(0) (:330-334)."""
+    leg_size, slicing = int(leg_size), int(slicing)
+    if leg_size == 0:
+        raise ValueError("leg_size or slicing is zero!")
+    if leg_size < slicing:
+        sl = [slice(i, i + 1) for i in range(leg_size)]
+        return sl, len(sl)
+    if slicing <= 1:
+        return [slice(0, leg_size)], 1
+    bs1, bs2 = leg_size // slicing + 1, leg_size // slicing
+    n1 = (leg_size - bs2 * slicing) // (bs1 - bs2)
+    sl = [slice(i, i + bs1) for i in range(0, n1 * bs1, bs1)]
+    sl += [slice(i, i + bs2) for i in range(n1 * bs1, leg_size, bs2)]
+    return sl, len(sl)
+
+
+def contract_G(A, PX, UM, PY):
+    """G_{T,X,Y,k} = sum A_{T,a,b,i} PX_{a,c,X} UM_{i,c,d,k} PY_{b,d,Y} (:379-384).
+
+    Pairwise order A.PX (2 chi^6), .UM over (i,c) (2 chi^7), .PY over (b,d)."""
+    W = np.tensordot(A, PX, axes=([1], [0]))                # (T,b,i,c,X)
+    W = np.tensordot(W, UM, axes=([2, 3], [0, 1]))          # (T,b,X,d,k)
+    W = np.tensordot(W, PY, axes=([1, 3], [0, 1]))          # (T,X,k,Y)
+    return np.transpose(W, (0, 1, 3, 2))
+
+
+def contract_H(VMH, Px, D, Py):
+    """H_{k,t,x,y} = sum VMH_{k,j,a,b} Px_{x,a,c} D_{j,t,c,d} Py_{y,b,d} (:400-405).
+
+    Pairwise order VMH.Px (2 chi^6), .D over (j,c) (2 chi^7), .Py over (b,d)."""
+    W = np.tensordot(VMH, Px, axes=([2], [1]))              # (k,j,b,x,c)
+    W = np.tensordot(W, D, axes=([1, 4], [0, 2]))           # (k,b,x,t,d)
+    W = np.tensordot(W, Py, axes=([1, 4], [1, 2]))          # (k,x,t,y)
+    return np.transpose(W, (0, 2, 1, 3))
+
+
+def coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY, slicing=100):
+    """trg/ATRG_3d_new.py:357-438.  The reference's slice loops only bound
+    memory; the sums are evaluated whole here."""
+    nT, nt = A.shape[0], D.shape[1]
+    nX, nY = PX.shape[2], PY.shape[2]
+    nx, ny = Px.shape[0], Py.shape[0]
+    nk = UM.shape[3]
+    G = contract_G(A, PX, UM, PY)
+    H = contract_H(VMH, Px, D, Py)
+    kG = min(nT * nX * nY, nk)
+    kH = min(nk, nt * nx * ny)
+    UG, sG, VGH = svd(G, shape=[[0, 1, 2], [3]], k=kG, truncate_eps=truncate_eps, degeneracy_eps=degeneracy_eps)
+    UH, sH, VHH = svd(H, shape=[[0], [1, 2, 3]], k=kH, truncate_eps=truncate_eps, degeneracy_eps=degeneracy_eps)
+    K = (sG[:, None] * VGH * sM[None, :]) @ (UH * sH[None, :])
+    kK = min(*K.shape)
+    UK, sK, VKH = svd(K, shape=[[0], [1]], k=kK, truncate_eps=truncate_eps, degeneracy_eps=degeneracy_eps)
+    U = np.tensordot(UG, UK, axes=([3], [0]))
+    VH = np.tensordot(VKH, VHH, axes=([1], [0]))
+    return U, sK, VH
+
+
+def atrg_pure_tensor(T, Dcut, direction):
+    """trg/ATRG_3d_new.py:441-482."""
+    T = leg_transposition(T, "transpose", direction)
+    A, D = T.U, T.VH
+    B = T.s[:, None, None, None] * T.VH
+    C = T.U * T.s[None, None, None, :]
+    UM, sM, VMH = leg_swapping(B, C, Dcut, ispure=True)
+    Px, PX, Py, PY = squeezer(A, UM, sM, VMH, D, Dcut)
+    U, s, VH = coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY, slicing=Dcut)
+    T = T.update(U, s, VH)
+    return leg_transposition(T, "restore", direction)
+
+
+def normalization(T):
+    """trg/ATRG_3d_new.py:954-958: c = |Tr T|, s /= c."""
+    c = float(np.abs(T.trace()))
+    T.s = T.s / c
+    return T, c
+
+
+def exec_pure_tensor_renorm(T, Dcut, direction, rgstep):
+    """trg/ATRG_3d_new.py:967-972."""
+    T = atrg_pure_tensor(T, Dcut, direction)
+    key = f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
+    T, c = normalization(T)
+    T.normal_const[key] = math.log(c) / 2 ** (sum(rgstep.values()))
+    return T
+
+
+def pure_tensor(T, Dcut, TOT_RGSTEPS):
+    """trg/ATRG_3d_new.py:974-1039: cycle T, X, Y until the totals are reached."""
+    rgstep = {"X": 0, "Y": 0, "T": 0}
+    key = f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
+    T, c = normalization(T)
+    T.normal_const[key] = math.log(c)
+    total = sum(TOT_RGSTEPS.values())
+    for direction in itertools.cycle("TXY"):
+        if sum(rgstep.values()) >= total:
+            break
+        if rgstep[direction] < TOT_RGSTEPS[direction]:
+            rgstep[direction] += 1
+            T = exec_pure_tensor_renorm(T, Dcut, direction, rgstep)
+    return T
+
+
+def ln_Z_over_V(T, Dcut, TOT_RGSTEPS):
+    """measurement/Potts3_model_3d.py:32-54: sum of normal_const + ln Tr T / V."""
+    T = pure_tensor(T, Dcut, TOT_RGSTEPS)
+    V = 2 ** sum(TOT_RGSTEPS.values())
+    return float(np.real(np.sum(T.get_normalization_const()) + np.log(T.trace()) / V)), T

@@ -1,0 +1,189 @@
+"""CPU tests that pin the NumPy oracle against closed forms, brute force and LAPACK."""
+import warnings
+
+import numpy as np
+import pytest
+
+from oracle import ATRG_3d_new as at
+from oracle import HOTRG_2d as h2
+from oracle import HOTRG_3d as h3
+from oracle import exact, models, philox, randomized_svd, truncated_svd
+
+warnings.filterwarnings("ignore")
+
+ONSAGER = {0.40: 0.879363820774948, 0.44: 0.928728540456889, 0.48: 0.991524464466794}
+
+
+@pytest.mark.parametrize("beta", sorted(ONSAGER))
+def test_onsager_two_quadratures(beta):
+    a = exact.onsager_lnZ(beta)
+    b = exact.onsager_lnZ_double(beta)
+    assert abs(a - ONSAGER[beta]) < 1e-13
+    assert abs(b - ONSAGER[beta]) < 1e-12
+
+
+def test_brute_force_against_transfer_matrix():
+    # 2D torus Ly x Lx via a row transfer matrix: independent of enumeration
+    beta, L = 0.37, 3
+    confs = ((np.arange(2 ** L)[:, None] >> np.arange(L)[None, :]) & 1) * 2 - 1
+    intra = np.sum(confs * np.roll(confs, 1, axis=1), axis=1)
+    inter = confs @ confs.T
+    Tm = np.exp(beta * (inter + 0.5 * (intra[:, None] + intra[None, :])))
+    lnZ = np.log(np.trace(np.linalg.matrix_power(Tm, L))) / (L * L)
+    assert abs(lnZ - exact.brute_lnZ_ising2d(beta, L, L)) < 1e-13
+
+
+def test_hotrg2d_untruncated_equals_brute_force():
+    beta = 0.44
+    T = models.ising2d_tensor(beta)
+    assert abs(h2.ln_Z_over_V(T, 16, 2, 2) - exact.brute_lnZ_ising2d(beta, 4, 4)) < 1e-13
+
+
+def test_hotrg2d_zero_padding_equals_growth():
+    T = models.ising2d_tensor(0.41)
+    a = h2.ln_Z_over_V(T, 8, 4, 4)
+    b = h2.ln_Z_over_V(models.zero_pad(T, 8), 8, 4, 4)
+    assert abs(a - b) < 1e-12
+
+
+def test_hotrg2d_beta0_is_ln2():
+    assert abs(h2.ln_Z_over_V(models.ising2d_tensor(0.0), 8, 6, 6) - np.log(2)) < 1e-13
+
+
+def test_hotrg2d_chi16_onsager():
+    v = h2.ln_Z_over_V(models.ising2d_tensor(0.44), 16, 15, 15)
+    assert abs(v - ONSAGER[0.44]) < 1e-6
+
+
+def test_hotrg2d_projector_properties():
+    T = models.ising2d_tensor(0.44)
+    for _ in range(3):
+        T = h2.new_pure_tensor(T, 8, "Y")
+        T, _ = h2.__tensor_normalization__(T)
+        T = h2.new_pure_tensor(T, 8, "X")
+        T, _ = h2.__tensor_normalization__(T)
+    MM1, MM2 = h2.environments(T)
+    # environment = M M^T with M the left half-network, built independently by einsum
+    ref = np.einsum("aijk,blkm,cijn,dlnm->abcd", T, T, T, T).reshape(MM1.shape)
+    assert np.allclose(MM1, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    assert np.allclose(MM1, MM1.T) and np.linalg.eigvalsh(MM1).min() > -1e-12 * np.abs(MM1).max()
+    U, _ = h2.__contraction_projector__(T, 8)
+    Um = U.reshape(-1, U.shape[2])
+    assert np.allclose(Um.T @ Um, np.eye(Um.shape[1]), atol=1e-12)
+    # merge equals the one-shot einsum
+    ref = np.einsum("acke,bdel,abi,cdj->ijkl", T, T, U, U)
+    assert np.allclose(h2.merge(T, T, U, U, nslice=3), ref, atol=1e-12 * np.abs(ref).max())
+
+
+def test_hotrg3d_untruncated_equals_brute_force():
+    beta = 0.22
+    assert abs(h3.ln_Z_over_V(models.ising3d_tensor(beta), 16, 1, 1, 1) - exact.brute_lnZ_ising3d(beta)) < 1e-13
+
+
+def test_hotrg3d_merge_and_env_match_einsum():
+    rng = np.random.default_rng(0)
+    T = rng.standard_normal((3, 3, 2, 2, 2, 2))
+    U1 = np.linalg.qr(rng.standard_normal((9, 4)))[0].reshape(3, 3, 4)
+    U2 = np.linalg.qr(rng.standard_normal((4, 3)))[0].reshape(2, 2, 3)
+    ref = np.einsum("ijklez,mnopzf,ima,jnb,koc,lpd->abcdef", T, T, U1, U1, U2, U2)
+    assert np.allclose(h3.merge(T, T, U1, U1, U2, U2, nslice=2), ref, atol=1e-12)
+    ref = np.einsum("efaghm,ijbkml,efcghn,ijdknl->abcd", T, T, T, T).reshape(4, 4)
+    assert np.allclose(h3.environment(T, 2), ref, atol=1e-12)
+
+
+def test_hotrg3d_beta0():
+    assert abs(h3.ln_Z_over_V(models.ising3d_tensor(0.0), 4, 2, 2, 2) - np.log(2)) < 1e-13
+
+
+def test_atrg_ising_untruncated_equals_brute_force():
+    at.set_rsvd_params()
+    beta = 0.22
+    v, _ = at.ln_Z_over_V(models.to_atrg(models.ising3d_tensor(beta), 64), 64, {"X": 1, "Y": 1, "T": 1})
+    assert abs(v - exact.brute_lnZ_ising3d(beta)) < 1e-12
+
+
+@pytest.mark.parametrize("steps,shape", [({"X": 0, "Y": 0, "T": 1}, (1, 1, 2)),
+                                         ({"X": 1, "Y": 0, "T": 1}, (2, 1, 2))])
+def test_atrg_potts_untruncated_equals_brute_force(steps, shape):
+    at.set_rsvd_params(k_factor=1, p=0, q=2)
+    try:
+        v, _ = at.ln_Z_over_V(models.to_atrg(models.potts3d_tensor(0.55), 243), 243, steps)
+    finally:
+        at.set_rsvd_params()
+    assert abs(v - exact.brute_lnZ_potts3d(0.55, shape=shape)) < 1e-12
+
+
+def test_atrg_beta0_potts_is_ln3():
+    v, _ = at.ln_Z_over_V(models.to_atrg(models.potts3d_tensor(0.0), 16), 16, {"X": 2, "Y": 2, "T": 2})
+    assert abs(v - np.log(3)) < 1e-12
+
+
+def test_atrg_G_H_match_einsum():
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((2, 3, 2, 4)); UM = rng.standard_normal((4, 2, 3, 5))
+    PX = rng.standard_normal((3, 2, 3)); PY = rng.standard_normal((2, 3, 2))
+    ref = np.einsum("Tabi,acX,icdk,bdY->TXYk", A, PX, UM, PY)
+    assert np.allclose(at.contract_G(A, PX, UM, PY), ref)
+    VMH = rng.standard_normal((5, 4, 3, 2)); Px = rng.standard_normal((3, 3, 2))
+    D = rng.standard_normal((4, 2, 2, 3)); Py = rng.standard_normal((2, 2, 3))
+    ref = np.einsum("kjab,xac,jtcd,ybd->ktxy", VMH, Px, D, Py)
+    assert np.allclose(at.contract_H(VMH, Px, D, Py), ref)
+
+
+def test_truncate_hand_cases():
+    s = np.array([1.0, 0.5, 0.5, 0.1, 1e-12])
+    assert truncated_svd.truncate(s, 3, 0, 0) == (3, 0)
+    assert truncated_svd.truncate(s, 9, 1e-3, 1e-10) == (5, 0)
+    # k = 2 would split the 0.5 pair: utility/truncated_svd.py:52-54 steps down to chi = 1
+    chi, err = truncated_svd.truncate(s, 2, 1e-3, 1e-10)
+    assert chi == 1 and abs(err - np.sqrt(np.sum(s[1:] ** 2) / np.sum(s ** 2))) < 1e-15
+    # k = 3 closes the pair and is kept
+    assert truncated_svd.truncate(s, 3, 1e-3, 1e-10)[0] == 3
+    chi, _ = truncated_svd.truncate(s, 5, 0.0, 1e-10)      # tail norm 1e-12 < eps -> drop it
+    assert chi == 4
+
+
+def test_truncated_svd_wrappers_reconstruct():
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((3, 4, 5))
+    u, s, vh = truncated_svd.svd(A, shape=[[0, 2], [1]], k=4, truncate_eps=1e-14, degeneracy_eps=0)
+    R = np.einsum("aci,i,ib->abc", u, s, vh)
+    assert np.allclose(R, A)
+    H = rng.standard_normal((6, 6)); H = H @ H.T
+    e, v = truncated_svd.eigh(H.reshape(2, 3, 2, 3), shape=[[0, 1], [2, 3]], k=6, degeneracy_eps=0, truncate_eps=1e-15)
+    assert np.allclose(e, np.sort(np.linalg.eigvalsh(H))[::-1])
+
+
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32_10
+    out = philox.philox4x32_10(0, 0, 0, 0, 0, 0)
+    assert [int(x) for x in out] == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    m = 0xFFFFFFFF
+    out = philox.philox4x32_10(m, m, m, m, m, m)
+    assert [int(x) for x in out] == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    out = philox.philox4x32_10(0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344, 0xa4093822, 0x299f31d0)
+    assert [int(x) for x in out] == [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_gaussian_moments():
+    z = philox.gaussian(200000, seed=1)
+    assert abs(z.mean()) < 0.01 and abs(z.var() - 1) < 0.01
+
+
+def test_rsvd_exact_when_sketch_covers_rank():
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((40, 6)) @ rng.standard_normal((6, 30))
+    u, s, vh = randomized_svd.rsvd(A, k=6, n_oversamples=4, n_power_iter=1)
+    assert np.allclose(s, np.linalg.svd(A, compute_uv=False)[:6])
+    assert np.allclose((u * s) @ vh, A)
+    u, s, vh = randomized_svd.rsvd2(A, k=6, n_oversamples=2)
+    assert np.allclose((u * s) @ vh, A)
+
+
+def test_leg_swap_rsvd_matches_dense_svd():
+    rng = np.random.default_rng(4)
+    B = rng.standard_normal((4, 3, 5)); C = rng.standard_normal((5, 4, 3))
+    M = np.einsum("bit,tjg->igjb", B, C).reshape(9, 16)
+    UM, sM, VMH = at.rsvd_for_ATRG_leg_swapping(B, C, chi_k=9, p=0, q=2, eps=1e-12, seed=7)
+    assert np.allclose(sM, np.linalg.svd(M, compute_uv=False)[:len(sM)])
+    assert np.allclose(np.einsum("igk,k,kjb->igjb", UM, sM, VMH).reshape(9, 16), M)
