@@ -1,0 +1,143 @@
+"""GPU tests of the individual libtrg kernels (through the C ABI via ops)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import philox
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(x, gpu):
+    return torch.as_tensor(np.ascontiguousarray(x)).to(gpu)
+
+
+def test_device_info(gpu):
+    import ctypes
+    from trg_backup_amd import _lib
+    lib = _lib.load()
+    cus, lds, ws = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert lib.trg_device_info(0, ctypes.byref(cus), ctypes.byref(lds), ctypes.byref(ws)) == 0
+    print(f"\n[device] CUs={cus.value} LDS/block={lds.value} wave={ws.value}", flush=True)
+    assert ws.value == 64
+
+
+@pytest.mark.parametrize("dtype,compute,rtol", [(torch.float64, None, 1e-13), (torch.float32, None, 2e-6),
+                                                (torch.float32, "bf16", 2e-2)])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (17, 33, 9), (128, 128, 16), (300, 257, 130), (64, 1000, 200)])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_gemm_layouts(gpu, dtype, compute, rtol, shape, ta, tb):
+    from trg_backup_amd import ops
+    M, N, K = shape
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    Ad, Bd = _dev(A, gpu).to(dtype), _dev(B, gpu).to(dtype)
+    Av = Ad.t() if ta else Ad
+    Bv = Bd.t() if tb else Bd
+    C = ops.gemm(Av, Bv, compute=compute)
+    ref = (A.T if ta else A) @ (B.T if tb else B)
+    err = np.abs(C.double().cpu().numpy() - ref).max() / (np.abs(A).max() * np.abs(B).max() * K)
+    assert err < rtol, err
+
+
+def test_gemm_batched_alpha_beta(gpu):
+    from trg_backup_amd import ops
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((5, 70, 40))
+    B = rng.standard_normal((40, 90))
+    C0 = rng.standard_normal((5, 70, 90))
+    Cd = _dev(C0, gpu)
+    ops.gemm(_dev(A, gpu), _dev(B, gpu)[None], Cd, alpha=0.5, beta=-2.0)
+    ref = 0.5 * A @ B - 2.0 * C0
+    assert np.allclose(Cd.cpu().numpy(), ref, atol=1e-11)
+
+
+def test_copy_permute_cast(gpu):
+    from trg_backup_amd import ops
+    x = np.random.default_rng(1).standard_normal((3, 4, 5, 6, 7))
+    xd = _dev(x, gpu)
+    y = ops.permute(xd, (4, 2, 0, 3, 1))
+    assert np.array_equal(y.cpu().numpy(), np.transpose(x, (4, 2, 0, 3, 1)))
+    z = ops.cast(xd, torch.float32)
+    assert np.array_equal(z.cpu().numpy(), x.astype(np.float32))
+    w = ops.permute(z, (1, 0, 2, 4, 3), dtype=torch.float64)
+    assert np.array_equal(w.cpu().numpy(), np.transpose(x.astype(np.float32), (1, 0, 2, 4, 3)).astype(np.float64))
+
+
+def test_maxabs_trace_normalize_lnz(gpu):
+    from trg_backup_amd import ops
+    rng = np.random.default_rng(2)
+    T = rng.standard_normal((3, 6, 6, 6, 6))
+    T[1, 2, 3, 4, 5] = -50.0
+    Td = _dev(T, gpu)
+    c = ops.maxabs(Td, batch=3).cpu().numpy()
+    assert np.array_equal(c, np.abs(T).reshape(3, -1).max(axis=1))
+    tr = ops.trace_pairs(Td, batch=3).cpu().numpy()
+    assert np.allclose(tr, np.einsum("baabb->b", T.reshape(3, 6, 6, 6, 6)) if False else
+                       np.array([np.einsum("aabb", T[b]) for b in range(3)]), atol=1e-12)
+    lnf = torch.zeros((3, 4), dtype=torch.float64, device=gpu)
+    ops.normalize_(Td, ops.maxabs(Td, batch=3), lnf, step=2, batch=3, lnf_stride=4)
+    assert np.allclose(Td.cpu().numpy(), T / c[:, None, None, None, None], rtol=1e-15)
+    assert np.allclose(lnf.cpu().numpy()[:, 2], np.log(c) / 4)
+    out = ops.lnz(lnf, _dev(np.array([2.0, 3.0, 4.0]), gpu), 8.0, batch=3).cpu().numpy()
+    assert np.allclose(out, np.log(c) / 4 + np.log([2.0, 3.0, 4.0]) / 8)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 32, 64, 128, 200])
+def test_syevj_matches_lapack(gpu, n):
+    from trg_backup_amd import ops
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n))
+    A = X @ X.T * np.exp(-np.arange(n) / 6.0)[None, :]
+    A = 0.5 * (A + A.T)
+    w, V, info = ops.syevj(_dev(np.stack([A, 2 * A]), gpu), batch=2)
+    w, V = w.cpu().numpy(), V.cpu().numpy()
+    we = np.linalg.eigvalsh(A)
+    assert np.allclose(w[0], we, atol=1e-13 * np.abs(we).max())
+    assert np.allclose(w[1], 2 * we, atol=2e-13 * np.abs(we).max())
+    R = A @ V[0] - V[0] * w[0][None, :]
+    assert np.abs(R).max() < 1e-12 * np.abs(we).max()
+    assert np.allclose(V[0].T @ V[0], np.eye(n), atol=1e-13)
+    assert int(info.cpu().numpy().max()) == 0
+
+
+def test_chol_inv(gpu):
+    from trg_backup_amd import ops
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((300, 96))
+    G = X.T @ X
+    R, info = ops.chol_inv(_dev(G, gpu))
+    R = R[0].cpu().numpy()
+    Q = X @ R
+    assert np.allclose(Q.T @ Q, np.eye(96), atol=1e-12)
+    assert np.allclose(np.tril(R, -1), 0)
+    assert int(info.cpu().numpy()[0]) == 0
+
+
+def test_gaussian_matches_oracle_philox(gpu):
+    from trg_backup_amd import ops
+    g = ops.gaussian((4, 1000), seed=1, device=gpu).cpu().numpy()
+    ref = philox.standard_normal((4, 1000), seed=1)
+    assert np.allclose(g, ref, rtol=1e-13, atol=1e-13)
+    g2 = ops.gaussian((100,), seed=123456789012345, dtype=torch.float32, device=gpu, stream_id=3).cpu().numpy()
+    assert np.allclose(g2, philox.gaussian(100, 123456789012345, stream=3), rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("n,k", [(64, 16), (256, 16), (1024, 32), (4096, 64)])
+def test_eigh_topk(gpu, n, k):
+    from trg_backup_amd import ops
+    rng = np.random.default_rng(n + k)
+    lam = np.exp(-np.arange(n) / (k / 3.0))
+    Qr, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (Qr * lam[None, :]) @ Qr.T
+    A = 0.5 * (A + A.T)
+    l = min(n, 2 * k)
+    w, V, eps, info = ops.eigh_topk(_dev(A, gpu), k, l, 0 if l == n else 12)
+    w, V, eps = w[0].cpu().numpy(), V[0].cpu().numpy(), float(eps[0].cpu().numpy())
+    assert np.allclose(w, lam[:k][::-1], rtol=1e-10, atol=1e-14)
+    P = V @ V.T
+    P0 = Qr[:, :k] @ Qr[:, :k].T
+    assert np.linalg.norm(P - P0) < 1e-9
+    assert abs(eps - lam[k:].sum()) < 1e-10 * lam.sum()
+    assert int(info.cpu().numpy()[0]) == 0

@@ -1,0 +1,86 @@
+"""ctypes binding of libtrg.so (the C ABI declared in include/trg.h).
+
+The library is built in-tree by ``trg_backup_amd.build``.  There is no CPU
+fallback: every op of this package runs on the GPU through this library or
+raises ``TrgError``.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtrg.so")
+
+F32, F64, BF16 = 0, 1, 2
+
+c_int, c_ll, c_dbl, c_vp, c_ull, c_uint = (ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p,
+                                           ctypes.c_ulonglong, ctypes.c_uint)
+P_LL = ctypes.POINTER(ctypes.c_longlong)
+
+# name -> argument types (restype is int unless listed in _RESTYPES)
+SIGNATURES = {
+    "trg_version": [],
+    "trg_status_string": [c_int],
+    "trg_last_hip_error": [],
+    "trg_device_info": [c_int, c_vp, c_vp, c_vp],
+    "trg_gemm": [c_int, c_int, c_int, c_int, c_int, c_int, c_dbl,
+                 c_vp, c_ll, c_ll, c_ll, c_vp, c_ll, c_ll, c_ll,
+                 c_dbl, c_vp, c_ll, c_ll, c_ll, c_vp],
+    "trg_copy_strided": [c_int, c_int, c_int, P_LL, c_vp, P_LL, c_vp, P_LL, c_vp],
+    "trg_fill": [c_int, c_ll, c_dbl, c_vp, c_vp],
+    "trg_maxabs": [c_int, c_int, c_ll, c_vp, c_ll, c_vp, c_vp, c_int, c_vp],
+    "trg_trace": [c_int, c_int, c_int, P_LL, P_LL, c_vp, c_ll, c_vp, c_vp],
+    "trg_normalize": [c_int, c_int, c_ll, c_vp, c_ll, c_vp, c_vp, c_ll, c_int, c_vp, c_vp],
+    "trg_syevj_f64": [c_int, c_int, c_vp, c_ll, c_vp, c_vp, c_int, c_vp, c_vp],
+    "trg_chol_inv_f64": [c_int, c_int, c_vp, c_ll, c_dbl, c_vp, c_vp, c_vp],
+    "trg_gaussian": [c_int, c_int, c_ll, c_ull, c_uint, c_uint, c_ull, c_vp, c_ll, c_vp],
+    "trg_eigh_topk_workspace": [c_int, c_int, c_int],
+    "trg_eigh_topk_f64": [c_int, c_int, c_int, c_int, c_int, c_vp, c_ll, c_vp, c_vp, c_vp, c_ull,
+                          c_vp, c_ll, c_vp, c_vp],
+    "trg_select_isometry": [c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "trg_scale_rows": [c_int, c_int, c_ll, c_ll, c_ll, c_vp, c_ll, c_vp, c_ll, c_int, c_vp],
+    "trg_lnz": [c_int, c_int, c_vp, c_ll, c_vp, c_dbl, c_vp, c_vp],
+}
+_RESTYPES = {"trg_version": ctypes.c_char_p, "trg_status_string": ctypes.c_char_p,
+             "trg_eigh_topk_workspace": c_ll}
+
+
+class TrgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libtrg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TrgError(f"{LIB_PATH} is not built; run `python -m trg_backup_amd.build` "
+                       "(or __graft_entry__.build()) first")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, c_int)
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call an entry point and raise TrgError on a non-zero status."""
+    lib = load()
+    st = getattr(lib, name)(*args)
+    if st != 0:
+        msg = lib.trg_status_string(st).decode()
+        extra = ""
+        if st == 6:
+            extra = f" (hip error {lib.trg_last_hip_error()})"
+        raise TrgError(f"{name} failed: {msg}{extra}")
+    return st
+
+
+def ll_array(vals):
+    arr = (ctypes.c_longlong * max(1, len(vals)))(*[int(v) for v in vals])
+    return arr

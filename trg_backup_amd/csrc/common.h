@@ -1,0 +1,31 @@
+// Shared helpers for the libtrg HIP sources (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/trg.h"
+
+namespace trg {
+
+extern thread_local int g_last_hip_error;
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Launch-error check after a kernel launch: returns TRG_HIP_ERROR and records the code.
+inline int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_last_hip_error = static_cast<int>(e);
+        return TRG_HIP_ERROR;
+    }
+    return TRG_OK;
+}
+
+inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
+
+}  // namespace trg
+
+#define TRG_TRY(expr)                         \
+    do {                                      \
+        int _st = (expr);                     \
+        if (_st != TRG_OK) return _st;        \
+    } while (0)

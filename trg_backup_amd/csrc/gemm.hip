@@ -1,0 +1,409 @@
+// Batched strided GEMM on CDNA4 matrix cores (gfx950).
+//
+//   C[b] = alpha * A[b] * B[b] + beta * C[b]
+//
+// Three instantiation families share one tiling scheme:
+//   * fp32 storage, fp32 MFMA   v_mfma_f32_32x32x2_f32   (32x32 block, k = 2)
+//   * fp32 storage, bf16 MFMA   v_mfma_f32_32x32x16_bf16 (operands rounded to
+//     bf16 while they are staged into LDS, fp32 accumulation)
+//   * fp64 storage, fp64 MFMA   v_mfma_f64_16x16x4_f64   (16x16 block, k = 4)
+//
+// A workgroup owns a BM x BN tile of C; its waves each own a WM x WN sub-tile
+// made of MFMA blocks.  K is walked in BK slices: the next slice is fetched
+// from HBM into registers while the current one is consumed from LDS.
+// Operand strides are arbitrary as long as one of the two strides of each
+// operand is 1, so leg permutations of tensors become index maps here
+// instead of separate copies whenever the contiguous leg allows it.
+#include "common.h"
+
+namespace trg {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct GemmArgs {
+    int M, N, K;
+    const void *A;
+    long long sAb, rsA, csA;
+    const void *B;
+    long long sBb, rsB, csB;
+    void *C;
+    long long sCb, rsC, csC;
+    double alpha, beta;
+    int tiles_n;   // number of BN tiles along N
+    int tiles_m;   // number of BM tiles along M
+};
+
+// ---------------------------------------------------------------------------
+// MFMA traits: operand lane map and accumulator lane map.
+// ---------------------------------------------------------------------------
+template <typename T> struct Mfma;
+
+template <> struct Mfma<float> {
+    static constexpr int MB = 32, KB = 2, NACC = 16;
+    typedef f32x16 acc_t;
+    __device__ static inline acc_t zero() {
+        acc_t a;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) a[i] = 0.f;
+        return a;
+    }
+    __device__ static inline void mma(acc_t &c, float a, float b) {
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    }
+    // operand: lane holds A[m = lane % 32][k = lane / 32]
+    __device__ static inline int op_mn(int lane) { return lane & 31; }
+    __device__ static inline int op_k(int lane) { return lane >> 5; }
+    // accumulator register r of lane -> (row, col) inside the 32x32 block
+    __device__ static inline int c_row(int lane, int r) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
+    __device__ static inline int c_col(int lane) { return lane & 31; }
+};
+
+template <> struct Mfma<double> {
+    static constexpr int MB = 16, KB = 4, NACC = 4;
+    typedef f64x4 acc_t;
+    __device__ static inline acc_t zero() {
+        acc_t a;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = 0.0;
+        return a;
+    }
+    __device__ static inline void mma(acc_t &c, double a, double b) {
+        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int op_mn(int lane) { return lane & 15; }
+    __device__ static inline int op_k(int lane) { return lane >> 4; }
+    __device__ static inline int c_row(int lane, int r) { return (lane >> 4) * 4 + r; }
+    __device__ static inline int c_col(int lane) { return lane & 15; }
+};
+
+// Tile index of this workgroup.  Workgroups are dispatched round-robin over
+// the 8 XCDs of the chip, each with its own L2; consecutive linear ids are
+// re-mapped so that each XCD walks a contiguous band of tiles (tiles in a
+// band share A rows / B columns, so the band stays L2-resident).
+__device__ static inline void tile_coords(const GemmArgs &g, int &tm, int &tn) {
+    const int total = g.tiles_m * g.tiles_n;
+    int lin = blockIdx.x;
+    if ((total & 7) == 0) {
+        const int per = total >> 3;
+        lin = (lin & 7) * per + (lin >> 3);
+    }
+    // group GROUP_M tile rows together so a band re-uses B columns
+    constexpr int GROUP_M = 8;
+    const int group_size = GROUP_M * g.tiles_n;
+    const int group = lin / group_size;
+    const int first_m = group * GROUP_M;
+    const int gm = min(g.tiles_m - first_m, GROUP_M);
+    const int in_group = lin - group * group_size;
+    tm = first_m + in_group % gm;
+    tn = in_group / gm;
+}
+
+// ---------------------------------------------------------------------------
+// fp32 / fp64 kernel.  LDS holds As[k][m] and Bs[k][n] (m / n fastest) so a
+// wave's operand fetch is one contiguous row segment per k.
+// ---------------------------------------------------------------------------
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool A_MCONT, bool B_NCONT>
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
+gemm_kernel(GemmArgs g) {
+    using MF = Mfma<T>;
+    using acc_t = typename MF::acc_t;
+    constexpr int WAVES_M = BM / WM;
+    constexpr int NT = (BM / WM) * (BN / WN) * 64;
+    constexpr int TM = WM / MF::MB;
+    constexpr int TN = WN / MF::MB;
+    constexpr int PAD = 16 / sizeof(T);
+    constexpr int AE = BM * BK / NT;
+    constexpr int BE = BN * BK / NT;
+    static_assert(AE * NT == BM * BK && BE * NT == BN * BK, "tile not divisible by threads");
+
+    __shared__ T As[BK][BM + PAD];
+    __shared__ T Bs[BK][BN + PAD];
+
+    int tm, tn;
+    tile_coords(g, tm, tn);
+    const int b = blockIdx.y;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const T *__restrict__ A = static_cast<const T *>(g.A) + (long long)b * g.sAb;
+    const T *__restrict__ B = static_cast<const T *>(g.B) + (long long)b * g.sBb;
+    T *__restrict__ C = static_cast<T *>(g.C) + (long long)b * g.sCb;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+
+    acc_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = MF::zero();
+
+    T ra[AE], rb[BE];
+
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < AE; ++i) {
+            const int e = tid + i * NT;
+            int mm, kk;
+            if (A_MCONT) { kk = e / BM; mm = e % BM; } else { mm = e / BK; kk = e % BK; }
+            const int m = m0 + mm, k = k0 + kk;
+            ra[i] = (m < g.M && k < g.K) ? A[(long long)m * g.rsA + (long long)k * g.csA] : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < BE; ++i) {
+            const int e = tid + i * NT;
+            int nn, kk;
+            if (B_NCONT) { kk = e / BN; nn = e % BN; } else { nn = e / BK; kk = e % BK; }
+            const int n = n0 + nn, k = k0 + kk;
+            rb[i] = (n < g.N && k < g.K) ? B[(long long)k * g.rsB + (long long)n * g.csB] : T(0);
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int i = 0; i < AE; ++i) {
+            const int e = tid + i * NT;
+            int mm, kk;
+            if (A_MCONT) { kk = e / BM; mm = e % BM; } else { mm = e / BK; kk = e % BK; }
+            As[kk][mm] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < BE; ++i) {
+            const int e = tid + i * NT;
+            int nn, kk;
+            if (B_NCONT) { kk = e / BN; nn = e % BN; } else { nn = e / BK; kk = e % BK; }
+            Bs[kk][nn] = rb[i];
+        }
+    };
+
+    load(0);
+    store();
+    __syncthreads();
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        const bool more = k0 + BK < g.K;
+        if (more) load(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += MF::KB) {
+            T a[TM], bb[TN];
+            const int kr = kk + MF::op_k(lane);
+#pragma unroll
+            for (int i = 0; i < TM; ++i) a[i] = As[kr][wm * WM + i * MF::MB + MF::op_mn(lane)];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) bb[j] = Bs[kr][wn * WN + j * MF::MB + MF::op_mn(lane)];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) MF::mma(acc[i][j], a[i], bb[j]);
+        }
+        __syncthreads();
+        if (more) {
+            store();
+            __syncthreads();
+        }
+    }
+
+    const T alpha = (T)g.alpha, beta = (T)g.beta;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < MF::NACC; ++r) {
+                const int m = m0 + wm * WM + i * MF::MB + MF::c_row(lane, r);
+                const int n = n0 + wn * WN + j * MF::MB + MF::c_col(lane);
+                if (m < g.M && n < g.N) {
+                    const long long idx = (long long)m * g.rsC + (long long)n * g.csC;
+                    T v = alpha * acc[i][j][r];
+                    if (g.beta != 0.0) v += beta * C[idx];
+                    C[idx] = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// fp32 storage, bf16 MFMA.  LDS holds As[m][k] and Bs[n][k] in bf16 (k
+// fastest) because v_mfma_f32_32x32x16_bf16 takes 8 consecutive k per lane:
+// lane holds A[m = lane % 32][k = 8 * (lane / 32) + j], j < 8.
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int BK, int WM, int WN, bool A_MCONT, bool B_NCONT>
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
+gemm_bf16_kernel(GemmArgs g) {
+    constexpr int WAVES_M = BM / WM;
+    constexpr int NT = (BM / WM) * (BN / WN) * 64;
+    constexpr int TM = WM / 32;
+    constexpr int TN = WN / 32;
+    constexpr int KP = BK + 8;  // row pitch in bf16 elements (16-byte aligned rows)
+    constexpr int AE = BM * BK / NT;
+    constexpr int BE = BN * BK / NT;
+    static_assert(BK % 16 == 0, "BK must be a multiple of 16");
+
+    __shared__ __attribute__((aligned(16))) __bf16 As[BM * KP];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[BN * KP];
+
+    int tm, tn;
+    tile_coords(g, tm, tn);
+    const int b = blockIdx.y;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const float *__restrict__ A = static_cast<const float *>(g.A) + (long long)b * g.sAb;
+    const float *__restrict__ B = static_cast<const float *>(g.B) + (long long)b * g.sBb;
+    float *__restrict__ C = static_cast<float *>(g.C) + (long long)b * g.sCb;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    float ra[AE], rb[BE];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < AE; ++i) {
+            const int e = tid + i * NT;
+            int mm, kk;
+            if (A_MCONT) { kk = e / BM; mm = e % BM; } else { mm = e / BK; kk = e % BK; }
+            const int m = m0 + mm, k = k0 + kk;
+            ra[i] = (m < g.M && k < g.K) ? A[(long long)m * g.rsA + (long long)k * g.csA] : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < BE; ++i) {
+            const int e = tid + i * NT;
+            int nn, kk;
+            if (B_NCONT) { kk = e / BN; nn = e % BN; } else { nn = e / BK; kk = e % BK; }
+            const int n = n0 + nn, k = k0 + kk;
+            rb[i] = (n < g.N && k < g.K) ? B[(long long)k * g.rsB + (long long)n * g.csB] : 0.f;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int i = 0; i < AE; ++i) {
+            const int e = tid + i * NT;
+            int mm, kk;
+            if (A_MCONT) { kk = e / BM; mm = e % BM; } else { mm = e / BK; kk = e % BK; }
+            As[mm * KP + kk] = (__bf16)ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < BE; ++i) {
+            const int e = tid + i * NT;
+            int nn, kk;
+            if (B_NCONT) { kk = e / BN; nn = e % BN; } else { nn = e / BK; kk = e % BK; }
+            Bs[nn * KP + kk] = (__bf16)rb[i];
+        }
+    };
+
+    load(0);
+    store();
+    __syncthreads();
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        const bool more = k0 + BK < g.K;
+        if (more) load(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            bf16x8 a[TM], bb[TN];
+            const int kr = kk + 8 * (lane >> 5);
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+                a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm * WM + i * 32 + (lane & 31)) * KP + kr]);
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                bb[j] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn * WN + j * 32 + (lane & 31)) * KP + kr]);
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            store();
+            __syncthreads();
+        }
+    }
+
+    const float alpha = (float)g.alpha, beta = (float)g.beta;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                const int n = n0 + wn * WN + j * 32 + (lane & 31);
+                if (m < g.M && n < g.N) {
+                    const long long idx = (long long)m * g.rsC + (long long)n * g.csC;
+                    float v = alpha * acc[i][j][r];
+                    if (g.beta != 0.0) v += beta * C[idx];
+                    C[idx] = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Host dispatch.
+// ---------------------------------------------------------------------------
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+static int launch_t(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
+    g.tiles_m = (int)cdiv(g.M, BM);
+    g.tiles_n = (int)cdiv(g.N, BN);
+    dim3 grid(g.tiles_m * g.tiles_n, batch);
+    dim3 block((BM / WM) * (BN / WN) * 64);
+    if (amc && bnc) hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, true>), grid, block, 0, s, g);
+    else if (amc) hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, false>), grid, block, 0, s, g);
+    else if (bnc) hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, true>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, false>), grid, block, 0, s, g);
+    return check_launch();
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+static int launch_bf16(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
+    g.tiles_m = (int)cdiv(g.M, BM);
+    g.tiles_n = (int)cdiv(g.N, BN);
+    dim3 grid(g.tiles_m * g.tiles_n, batch);
+    dim3 block((BM / WM) * (BN / WN) * 64);
+    if (amc && bnc) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, true>), grid, block, 0, s, g);
+    else if (amc) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, false>), grid, block, 0, s, g);
+    else if (bnc) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, true>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, false>), grid, block, 0, s, g);
+    return check_launch();
+}
+
+}  // namespace trg
+
+using namespace trg;
+
+extern "C" int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, double alpha,
+                        const void *A, long long sAb, long long rsA, long long csA,
+                        const void *B, long long sBb, long long rsB, long long csB,
+                        double beta, void *C, long long sCb, long long rsC, long long csC,
+                        void *stream) {
+    if (batch < 0 || M < 0 || N < 0 || K < 0) return TRG_BAD_ARG;
+    if (batch == 0 || M == 0 || N == 0) return TRG_OK;
+    if (!A || !B || !C) return TRG_BAD_ARG;
+    if (batch > 65535) return TRG_SHAPE;
+    const bool amc = (rsA == 1);
+    const bool bnc = (csB == 1);
+    if (!amc && csA != 1) return TRG_BAD_ARG;
+    if (!bnc && rsB != 1) return TRG_BAD_ARG;
+    hipStream_t s = as_stream(stream);
+    if (K == 0) {
+        // C = beta * C: a GEMM with an empty sum.  Use the kernel with K = 0 (loop skipped).
+    }
+    GemmArgs g{M, N, K, A, sAb, rsA, csA, B, sBb, rsB, csB, C, sCb, rsC, csC, alpha, beta, 0, 0};
+    const long long work = (long long)M * N;
+    if (dtype == TRG_F32 && compute == TRG_F32) {
+        if (work >= 256LL * 128 * 128) return launch_t<float, 128, 128, 16, 64, 64>(g, batch, amc, bnc, s);
+        return launch_t<float, 64, 64, 16, 32, 32>(g, batch, amc, bnc, s);
+    }
+    if (dtype == TRG_F32 && compute == TRG_BF16) {
+        if (work >= 256LL * 128 * 128) return launch_bf16<128, 128, 32, 64, 64>(g, batch, amc, bnc, s);
+        return launch_bf16<64, 64, 32, 32, 32>(g, batch, amc, bnc, s);
+    }
+    if (dtype == TRG_F64 && compute == TRG_F64) {
+        if (work >= 256LL * 128 * 128) return launch_t<double, 128, 128, 16, 64, 64>(g, batch, amc, bnc, s);
+        return launch_t<double, 64, 64, 16, 32, 32>(g, batch, amc, bnc, s);
+    }
+    return TRG_UNSUPPORTED;
+}

@@ -1,0 +1,415 @@
+// Float64 dense linear algebra for the isometry / decomposition steps.
+//
+//  * syevj: cyclic Jacobi eigensolver, one workgroup per matrix, parallel
+//    (round-robin) ordering so the n/2 rotations of a round are applied
+//    together; the matrix lives in LDS when it fits.  Jacobi keeps high
+//    relative accuracy, which matters for the small end of a spectrum that
+//    spans many decades.
+//  * chol_inv: shifted Cholesky of an l x l Gram matrix and the inverse of
+//    its upper-triangular factor: the building block of (shifted)
+//    CholeskyQR, used instead of Householder QR because it is two GEMMs
+//    plus an l^3 kernel.
+//  * eigh_topk: top-k eigenpairs of an n x n PSD matrix by block subspace
+//    iteration (block l > k), shift-regularised CholeskyQR3 orthonormalisation
+//    and a Rayleigh-Ritz step -- only the k kept eigenvectors of the
+//    environment are ever needed by HOTRG.
+#include <math.h>
+
+#include <mutex>
+
+#include "common.h"
+
+namespace trg {
+
+static int g_lds_limit[64];
+static std::once_flag g_lds_once[64];
+
+static int lds_limit() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+    std::call_once(g_lds_once[dev], [dev]() {
+        int v = 65536;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
+        g_lds_limit[dev] = v;
+    });
+    return g_lds_limit[dev];
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi eigensolver.  Work matrix W (n x n, row-major) either in LDS or in
+// the caller's global buffer.  V accumulates the rotations.
+// ---------------------------------------------------------------------------
+constexpr int JT = 256;   // threads per workgroup
+constexpr int JMAX = 256; // largest n
+
+__device__ static inline int rr_pos(int j, int r, int N) { return j == 0 ? 0 : ((j - 1 + r) % (N - 1)) + 1; }
+
+__global__ void __launch_bounds__(JT) syevj_kernel(int n, double *Ag, long long sA, double *wout, double *Vout,
+                                                   int max_sweeps, int *info, int use_lds) {
+    extern __shared__ double lds[];
+    __shared__ double cs[JMAX / 2], sn[JMAX / 2];
+    __shared__ int pp[JMAX / 2], qq[JMAX / 2];
+    __shared__ int rotated;
+    __shared__ double red[JT];
+    __shared__ double wv[JMAX];
+    __shared__ int rank[JMAX];
+
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    double *Ab = Ag + (long long)b * sA;
+    double *V = Vout + (long long)b * n * n;
+    double *W = use_lds ? lds : Ab;
+
+    // symmetrised copy W = (A + A^T)/2, V = I, Frobenius norm for the floor
+    double fro = 0.0;
+    for (int e = tid; e < n * n; e += JT) {
+        const int i = e / n, j = e % n;
+        const double v = 0.5 * (Ab[i * n + j] + Ab[j * n + i]);
+        fro += v * v;
+        V[e] = (i == j) ? 1.0 : 0.0;
+        if (use_lds) W[e] = v;
+    }
+    red[tid] = fro;
+    __syncthreads();
+    if (!use_lds) {
+        // in place: symmetrise the upper/lower halves (reads above used the old values)
+        for (int e = tid; e < n * n; e += JT) {
+            const int i = e / n, j = e % n;
+            if (i < j) {
+                const double v = 0.5 * (W[i * n + j] + W[j * n + i]);
+                W[i * n + j] = v;
+                W[j * n + i] = v;
+            }
+        }
+    }
+    for (int s = JT / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double floor_abs = 1e-300 + 1e-17 * sqrt(red[0]);
+    __syncthreads();
+
+    const int N = n + (n & 1);
+    const int npair = N / 2;
+    int sweep = 0;
+    for (; sweep < max_sweeps; ++sweep) {
+        if (tid == 0) rotated = 0;
+        __syncthreads();
+        for (int r = 0; r < N - 1; ++r) {
+            if (tid < npair) {
+                int p = rr_pos(tid, r, N), q = rr_pos(N - 1 - tid, r, N);
+                if (p > q) { const int t = p; p = q; q = t; }
+                double c = 1.0, s = 0.0;
+                if (q < n) {
+                    const double app = W[p * n + p], aqq = W[q * n + q], apq = W[p * n + q];
+                    const double thr = fmax(2.2e-16 * sqrt(fabs(app * aqq)), floor_abs);
+                    if (fabs(apq) > thr) {
+                        const double tau = (aqq - app) / (2.0 * apq);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = t * c;
+                        rotated = 1;
+                    }
+                }
+                cs[tid] = c; sn[tid] = s; pp[tid] = p; qq[tid] = q;
+            }
+            __syncthreads();
+            // rows: a_pj, a_qj <- c a_pj - s a_qj, s a_pj + c a_qj
+            for (int e = tid; e < npair * n; e += JT) {
+                const int i = e / n, j = e % n;
+                const double s = sn[i];
+                if (s != 0.0) {
+                    const int p = pp[i], q = qq[i];
+                    const double c = cs[i];
+                    const double ap = W[p * n + j], aq = W[q * n + j];
+                    W[p * n + j] = c * ap - s * aq;
+                    W[q * n + j] = s * ap + c * aq;
+                }
+            }
+            __syncthreads();
+            // columns of W and of V
+            for (int e = tid; e < npair * n; e += JT) {
+                const int i = e / n, j = e % n;
+                const double s = sn[i];
+                if (s != 0.0) {
+                    const int p = pp[i], q = qq[i];
+                    const double c = cs[i];
+                    const double ap = W[j * n + p], aq = W[j * n + q];
+                    W[j * n + p] = c * ap - s * aq;
+                    W[j * n + q] = s * ap + c * aq;
+                    const double vp = V[j * n + p], vq = V[j * n + q];
+                    V[j * n + p] = c * vp - s * vq;
+                    V[j * n + q] = s * vp + c * vq;
+                }
+            }
+            __syncthreads();
+        }
+        if (!rotated) break;
+        __syncthreads();
+    }
+    // eigenvalues, stable ascending rank, permuted eigenvector columns
+    for (int i = tid; i < n; i += JT) wv[i] = W[i * n + i];
+    __syncthreads();
+    for (int i = tid; i < n; i += JT) {
+        int rk = 0;
+        const double wi = wv[i];
+        for (int j = 0; j < n; ++j) {
+            const double wj = wv[j];
+            rk += (wj < wi) || (wj == wi && j < i);
+        }
+        rank[i] = rk;
+        wout[(long long)b * n + rk] = wi;
+    }
+    __syncthreads();
+    for (int e = tid; e < n * n; e += JT) W[e] = V[e];
+    __syncthreads();
+    for (int e = tid; e < n * n; e += JT) {
+        const int j = e / n, i = e % n;
+        V[j * n + rank[i]] = W[j * n + i];
+    }
+    if (tid == 0 && info) info[b] = (sweep >= max_sweeps) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Shifted Cholesky  R^T R = G + shift_rel * tr(G) * I  and  Rinv = R^{-1}.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) chol_inv_kernel(int l, double *Gg, long long sG, double shift_rel,
+                                                       double *Rinv, int *info, int use_lds) {
+    extern __shared__ double lds[];
+    __shared__ double red[256];
+    __shared__ double piv;
+    __shared__ int bad;
+    const int b = blockIdx.x, tid = threadIdx.x;
+    double *Gb = Gg + (long long)b * sG;
+    double *G = use_lds ? lds : Gb;
+    double *X = Rinv + (long long)b * l * l;
+    if (tid == 0) bad = 0;
+    double tr = 0.0;
+    for (int e = tid; e < l * l; e += 256) {
+        const int i = e / l, j = e % l;
+        const double v = 0.5 * (Gb[i * l + j] + Gb[j * l + i]);
+        if (use_lds) G[e] = v;
+        if (i == j) tr += v;
+    }
+    red[tid] = tr;
+    __syncthreads();
+    if (!use_lds) {
+        for (int e = tid; e < l * l; e += 256) {
+            const int i = e / l, j = e % l;
+            if (i < j) {
+                const double v = 0.5 * (G[i * l + j] + G[j * l + i]);
+                G[i * l + j] = v;
+                G[j * l + i] = v;
+            }
+        }
+    }
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double shift = shift_rel * red[0];
+    for (int i = tid; i < l; i += 256) G[i * l + i] += shift;
+    __syncthreads();
+    const double tiny = 1e-300 + 1e-30 * fabs(red[0]);
+    for (int j = 0; j < l; ++j) {
+        if (tid == 0) {
+            double d = G[j * l + j];
+            if (!(d > tiny)) {   // also catches NaN
+                bad = 1;
+                d = tiny > 0 ? tiny : 1e-300;
+            }
+            piv = sqrt(d);
+            G[j * l + j] = piv;
+        }
+        __syncthreads();
+        const double r = piv;
+        for (int i = j + 1 + tid; i < l; i += 256) G[j * l + i] /= r;
+        __syncthreads();
+        const int m = l - j - 1;
+        for (int e = tid; e < m * m; e += 256) {
+            const int a = j + 1 + e / m, c = j + 1 + e % m;
+            if (a <= c) G[a * l + c] -= G[j * l + a] * G[j * l + c];
+        }
+        __syncthreads();
+    }
+    // X = R^{-1}: back substitution, one column per thread
+    for (int c = tid; c < l; c += 256) {
+        for (int i = l - 1; i >= 0; --i) {
+            double v = 0.0;
+            if (i > c) {
+                X[i * l + c] = 0.0;
+                continue;
+            }
+            if (i == c) v = 1.0;
+            for (int k = i + 1; k <= c; ++k) v -= G[i * l + k] * X[k * l + c];
+            X[i * l + c] = v / G[i * l + i];
+        }
+    }
+    if (tid == 0 && info) info[b] = bad;
+}
+
+// Y[b] += shift_rel * t[b] * Q[b]   (t = trace of the operator)
+__global__ void axpy_scaled_kernel(long long n, double shift_rel, const double *t, const double *Q, long long sQ,
+                                   double *Y, long long sY) {
+    const int b = blockIdx.y;
+    const double a = shift_rel * t[b];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        Y[(long long)b * sY + i] += a * Q[(long long)b * sQ + i];
+}
+
+// eps[b] = t[b] - sum w[b][0..k)  (fixed order, one thread per instance)
+__global__ void discarded_kernel(int batch, int k, const double *t, const double *w, long long sw, double *eps) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    double s = 0.0;
+    for (int i = 0; i < k; ++i) s += w[(long long)b * sw + i];
+    eps[b] = t[b] - s;
+}
+
+// w[b][i] = theta[b][l-k+i]
+__global__ void tail_copy_kernel(int batch, int l, int k, const double *theta, double *w) {
+    const int b = blockIdx.y;
+    for (int i = threadIdx.x; i < k; i += blockDim.x) w[(long long)b * k + i] = theta[(long long)b * l + (l - k) + i];
+}
+
+__global__ void or_info_kernel(int batch, const int *src, int *dst) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < batch && src[b]) dst[b] = 1;
+}
+
+}  // namespace trg
+
+using namespace trg;
+
+extern "C" int trg_syevj_f64(int batch, int n, double *A, long long sA, double *w, double *V, int max_sweeps,
+                             int *info, void *stream) {
+    if (batch < 0 || n < 0 || max_sweeps < 1) return TRG_BAD_ARG;
+    if (batch == 0 || n == 0) return TRG_OK;
+    if (n > JMAX) return TRG_SHAPE;
+    if (!A || !w || !V) return TRG_BAD_ARG;
+    if (batch > 65535) return TRG_SHAPE;
+    const size_t bytes = (size_t)n * n * sizeof(double);
+    const int use_lds = (bytes + 16384 <= (size_t)lds_limit()) ? 1 : 0;
+    if (use_lds && bytes > 65536) {
+        if (hipFuncSetAttribute((const void *)syevj_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+            hipSuccess) {
+            g_last_hip_error = (int)hipGetLastError();
+            return TRG_HIP_ERROR;
+        }
+    }
+    hipLaunchKernelGGL(syevj_kernel, batch, JT, use_lds ? bytes : 0, as_stream(stream), n, A, sA, w, V, max_sweeps,
+                       info, use_lds);
+    return check_launch();
+}
+
+extern "C" int trg_chol_inv_f64(int batch, int l, double *G, long long sG, double shift_rel, double *Rinv, int *info,
+                                void *stream) {
+    if (batch < 0 || l < 0) return TRG_BAD_ARG;
+    if (batch == 0 || l == 0) return TRG_OK;
+    if (l > 256) return TRG_SHAPE;
+    if (!G || !Rinv) return TRG_BAD_ARG;
+    if (batch > 65535) return TRG_SHAPE;
+    const size_t bytes = (size_t)l * l * sizeof(double);
+    const int use_lds = (bytes + 8192 <= (size_t)lds_limit()) ? 1 : 0;
+    if (use_lds && bytes > 65536) {
+        if (hipFuncSetAttribute((const void *)chol_inv_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+            hipSuccess) {
+            g_last_hip_error = (int)hipGetLastError();
+            return TRG_HIP_ERROR;
+        }
+    }
+    hipLaunchKernelGGL(chol_inv_kernel, batch, 256, use_lds ? bytes : 0, as_stream(stream), l, G, sG, shift_rel, Rinv,
+                       info, use_lds);
+    return check_launch();
+}
+
+// Workspace (structure of arrays over the batch, doubles): Q, Y (batch*n*l
+// each), G, R, H, W (batch*l*l each), theta (batch*l), trace (batch); then
+// int flags (batch).
+extern "C" long long trg_eigh_topk_workspace(int batch, int n, int l) {
+    if (batch <= 0 || n <= 0 || l <= 0) return 0;
+    const long long dbl = (long long)batch * (2LL * n * l + 4LL * l * l + l + 1);
+    return dbl * (long long)sizeof(double) + 4LL * batch + 256;
+}
+
+// Orthonormalise X (batch of n x l) with shifted CholeskyQR3 (one shifted
+// pass, two plain passes; Fukaya et al., SIAM J. Sci. Comput. 42 (2020)).
+// Y is scratch of the same size; the result is left in X.
+static int cholqr3(int batch, int n, int l, double *X, double *Y, double *G, double *R, int *flags, int *info,
+                   hipStream_t s) {
+    const double shift0 = 11.0 * ((double)n * l + (double)l * (l + 1)) * 1.1102230246251565e-16;
+    const long long sX = (long long)n * l, sG = (long long)l * l;
+    double *src = X, *dst = Y;
+    for (int pass = 0; pass < 3; ++pass) {
+        // G = src^T src
+        TRG_TRY(trg_gemm(TRG_F64, TRG_F64, batch, l, l, n, 1.0, src, sX, 1, l, src, sX, l, 1, 0.0, G, sG, l, 1, s));
+        TRG_TRY(trg_chol_inv_f64(batch, l, G, sG, pass == 0 ? shift0 : 0.0, R, flags, s));
+        if (info) {
+            hipLaunchKernelGGL(or_info_kernel, (unsigned)cdiv(batch, 256), 256, 0, s, batch, flags, info);
+            TRG_TRY(check_launch());
+        }
+        // dst = src R^{-1}
+        TRG_TRY(trg_gemm(TRG_F64, TRG_F64, batch, n, l, l, 1.0, src, sX, l, 1, R, sG, l, 1, 0.0, dst, sX, l, 1, s));
+        double *t = src; src = dst; dst = t;
+    }
+    // an odd number of passes leaves the result in Y
+    long long dims[1] = {(long long)batch * n * l};
+    long long st[1] = {1};
+    return trg_copy_strided(TRG_F64, TRG_F64, 1, dims, src, st, X, st, s);
+}
+
+extern "C" int trg_eigh_topk_f64(int batch, int n, int k, int l, int q, const double *A, long long sA, double *w,
+                                 double *V, double *eps, unsigned long long seed, void *ws, long long ws_bytes,
+                                 int *info, void *stream) {
+    if (batch < 0 || n < 0 || k < 0 || l < 0 || q < 0) return TRG_BAD_ARG;
+    if (batch == 0 || n == 0 || k == 0) return TRG_OK;
+    if (k > l || l > n || l > 256) return TRG_SHAPE;
+    if (!A || !w || !V || !eps || !ws) return TRG_BAD_ARG;
+    if (ws_bytes < trg_eigh_topk_workspace(batch, n, l)) return TRG_OOM;
+    if (batch > 65535) return TRG_SHAPE;
+    hipStream_t s = as_stream(stream);
+    const long long sX = (long long)n * l, sG = (long long)l * l;
+    double *Q = static_cast<double *>(ws);
+    double *Y = Q + batch * sX;
+    double *G = Y + batch * sX;
+    double *R = G + batch * sG;
+    double *H = R + batch * sG;
+    double *W = H + batch * sG;
+    double *theta = W + batch * sG;
+    double *tr = theta + (long long)batch * l;
+    int *flags = reinterpret_cast<int *>(tr + batch);
+    if (info) TRG_TRY(trg_fill(3, batch, 0.0, info, s));
+
+    long long tdims[1] = {n}, tst[1] = {(long long)n + 1};
+    TRG_TRY(trg_trace(TRG_F64, batch, 1, tdims, tst, A, sA, tr, s));
+    // Gaussian start block, the same draw for every instance (batch-invariant)
+    TRG_TRY(trg_gaussian(TRG_F64, batch, sX, seed, 0u, 0u, 0ull, Q, sX, s));
+    TRG_TRY(cholqr3(batch, n, l, Q, Y, G, R, flags, info, s));
+    // (A + sigma I) keeps every iterate of full column rank; sigma is far
+    // below the kept eigenvalues and the Ritz step uses A itself.
+    const double shift_rel = 1e-13;
+    for (int it = 0; it < q; ++it) {
+        TRG_TRY(trg_gemm(TRG_F64, TRG_F64, batch, n, l, n, 1.0, A, sA, n, 1, Q, sX, l, 1, 0.0, Y, sX, l, 1, s));
+        const dim3 grid((unsigned)std::min<long long>(cdiv(sX, 256), 4096), batch);
+        hipLaunchKernelGGL(axpy_scaled_kernel, grid, 256, 0, s, sX, shift_rel, tr, Q, sX, Y, sX);
+        TRG_TRY(check_launch());
+        TRG_TRY(cholqr3(batch, n, l, Y, Q, G, R, flags, info, s));
+        long long dims[1] = {(long long)batch * sX}, st[1] = {1};
+        TRG_TRY(trg_copy_strided(TRG_F64, TRG_F64, 1, dims, Y, st, Q, st, s));
+    }
+    // Rayleigh-Ritz: H = Q^T A Q, H = W diag(theta) W^T
+    TRG_TRY(trg_gemm(TRG_F64, TRG_F64, batch, n, l, n, 1.0, A, sA, n, 1, Q, sX, l, 1, 0.0, Y, sX, l, 1, s));
+    TRG_TRY(trg_gemm(TRG_F64, TRG_F64, batch, l, l, n, 1.0, Q, sX, 1, l, Y, sX, l, 1, 0.0, H, sG, l, 1, s));
+    TRG_TRY(trg_syevj_f64(batch, l, H, sG, theta, W, 40, flags, s));
+    if (info) {
+        hipLaunchKernelGGL(or_info_kernel, (unsigned)cdiv(batch, 256), 256, 0, s, batch, flags, info);
+        TRG_TRY(check_launch());
+    }
+    // V = Q W[:, l-k:]  (eigenvectors of the k largest, ascending like eigh)
+    TRG_TRY(trg_gemm(TRG_F64, TRG_F64, batch, n, k, l, 1.0, Q, sX, l, 1, W + (l - k), sG, l, 1, 0.0, V,
+                     (long long)n * k, k, 1, s));
+    hipLaunchKernelGGL(tail_copy_kernel, dim3(1, batch), 256, 0, s, batch, l, k, theta, w);
+    TRG_TRY(check_launch());
+    hipLaunchKernelGGL(discarded_kernel, (unsigned)cdiv(batch, 256), 256, 0, s, batch, k, tr, w, (long long)k, eps);
+    return check_launch();
+}

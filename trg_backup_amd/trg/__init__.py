@@ -1,0 +1,1 @@
+"""Drop-in coarse-graining modules (trg/HOTRG_2d.py, trg/HOTRG_3d.py, trg/ATRG_3d_new.py)."""
