@@ -74,7 +74,8 @@ template <> struct Mfma<double> {
     }
     __device__ static inline int op_mn(int lane) { return lane & 15; }
     __device__ static inline int op_k(int lane) { return lane >> 4; }
-    __device__ static inline int c_row(int lane, int r) { return (lane >> 4) * 4 + r; }
+    // fp64 accumulators interleave rows: register r of lane holds row lane/16 + 4r
+    __device__ static inline int c_row(int lane, int r) { return (lane >> 4) + 4 * r; }
     __device__ static inline int c_col(int lane) { return lane & 15; }
 };
 

@@ -134,7 +134,7 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None):
     if N == 1:
         csB = 1 if rsB != 1 else csB
     if rsA != 1 and csA != 1:
-        A = A.contiguous() if False else contiguous(A)
+        A = contiguous(A)
         rsA, csA, sA = A.stride(1), A.stride(2), (A.stride(0) if bt > 1 else 0)
     if rsB != 1 and csB != 1:
         B = contiguous(B)
