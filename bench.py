@@ -1,0 +1,191 @@
+"""Benchmark: batched mixed-precision HOTRG 2D on one MI355X.
+
+Workload (one line of the JSON contract): 2D Ising HOTRG, chi = 64, 32
+couplings beta_i = 0.40 + 0.08 i / 31 batched into every launch, RG steps
+alternating Y/X.  The instances are grown untimed to the steady shape
+(bonds chi x chi); W warm-up steps and then K timed steps follow, each one a
+coarse-graining of all 32 instances along one axis in float32 on MFMA with
+float64 isometries (eigensolver), traces and log-norms.
+value = 32 * K / (timed seconds)  [RG steps/s, whole GPU].
+
+Also reported on the same line:
+  * roofline of the dominant kernel (the chi^7 GEMM of the merge), its
+    launches timed live with HIP events on the launching stream during the
+    timed region; algorithmic flops 2 M N K per launch;
+  * lnZ_err_vs_fp64: |ln Z/V (mixed) - ln Z/V (float64 path)| of full 40-step
+    solves for two couplings (untimed);
+  * plain_fp64_steps_per_s: one instance, float64 throughout, host sync per
+    step (the per-instance float64 GPU path);
+  * cpu_baseline: the NumPy float64 oracle timed on the host for one
+    steady-state chi = 64 RG step of one instance.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3     # MI355X dense fp32 matrix peak
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--chi", type=int, default=64)
+    ap.add_argument("--nbeta", type=int, default=32)
+    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(independent couplings shard trivially across GPUs, see DESIGN.md)"}))
+        return 2
+    import numpy as np
+    import torch
+
+    from oracle import models
+    from trg_backup_amd import _lib, hotrg, ops
+
+    _lib.load()
+    dev = torch.device("cuda:0")
+    chi, nb = args.chi, args.nbeta
+    betas = [0.40 + 0.08 * i / (nb - 1) for i in range(nb)] if nb > 1 else [0.44]
+    policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32)
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+
+    # ---- growth to the steady shape (untimed) ----
+    T = torch.stack([torch.as_tensor(models.ising2d_tensor(b)) for b in betas]).to(dev)
+    nsteps_total = 6 + args.warmup + args.steps
+    lnf = torch.zeros((nb, nsteps_total + 1), dtype=torch.float64, device=dev)
+    T = ops.contiguous(T)
+    hotrg.normalize_maxabs(T, lnf, 0)
+    step = 0
+    direction = "Y"
+    while T.shape[1] < chi or T.shape[3] < chi:
+        step += 1
+        T = hotrg.step2d(T, chi, direction, hotrg.FP64)
+        hotrg.normalize_maxabs(T, lnf, step)
+        direction = "X" if direction == "Y" else "Y"
+    T = ops.cast(T, torch.float32)
+    torch.cuda.synchronize()
+    log(f"steady shape {tuple(T.shape)} after {step} growth steps")
+
+    def rg_step(T, direction, st):
+        T = hotrg.step2d(T, chi, direction, hotrg.FP32)
+        hotrg.normalize_maxabs(T, lnf, st)
+        return T
+
+    for _ in range(args.warmup):
+        step += 1
+        T = rg_step(T, direction, step)
+        direction = "X" if direction == "Y" else "Y"
+    torch.cuda.synchronize()
+
+    rec = ops.GemmRecorder("merge2d_z2")
+    t0 = time.perf_counter()
+    with rec:
+        for _ in range(args.steps):
+            step += 1
+            T = rg_step(T, direction, step)
+            direction = "X" if direction == "Y" else "Y"
+        torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = nb * args.steps / dt
+    g = rec.summary()
+    achieved = g["avg_flops"] / (g["avg_ms"] * 1e-3) / 1e12
+    log(f"timed {args.steps} steps x {nb} instances in {dt:.3f} s; z2 gemm {g}")
+
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "gemm_merge2d_z2_traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    out = {
+        "metric": "RG steps/sec and p50/p99 solve latency at |dlnZ/V|<=1e-8 vs FP64 plain path",
+        "value": value, "unit": "RG steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"2D Ising HOTRG chi={chi}, {nb} beta in [0.40,0.48] batched, "
+                               "steady-state RG steps (Y/X alternating)",
+                   "chi": chi, "nbeta": nb, "policy": policy.describe(), "parallelism": "single GPU, batched"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "gemm_kernel<float,128,128,16,64,64> (merge chi^7 GEMM)",
+                     "launches": g["launches"], "avg_ms": g["avg_ms"], "flops_per_launch": g["avg_flops"]},
+    }
+    del T
+    torch.cuda.empty_cache()
+
+    # ---- per-instance float64 path with a host sync per step ----
+    if not args.no_plain:
+        T1 = torch.as_tensor(models.ising2d_tensor(0.44)).to(dev)[None].contiguous()
+        l1 = torch.zeros((1, 64), dtype=torch.float64, device=dev)
+        s, d = 0, "Y"
+        while T1.shape[1] < chi or T1.shape[3] < chi:
+            s += 1
+            T1 = hotrg.step2d(T1, chi, d, hotrg.FP64)
+            hotrg.normalize_maxabs(T1, l1, s)
+            d = "X" if d == "Y" else "Y"
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(2):
+            s += 1
+            t1 = time.perf_counter()
+            T1 = hotrg.step2d(T1, chi, d, hotrg.FP64)
+            hotrg.normalize_maxabs(T1, l1, s)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t1)
+            d = "X" if d == "Y" else "Y"
+        out["plain_fp64_steps_per_s"] = 1.0 / min(ts)
+        out["speedup_vs_plain_fp64"] = value / out["plain_fp64_steps_per_s"]
+        del T1
+        torch.cuda.empty_cache()
+
+    # ---- accuracy of the mixed policy against the float64 path (untimed) ----
+    if not args.no_validate:
+        vb = [0.40, 0.44]
+        T0 = torch.stack([torch.as_tensor(models.ising2d_tensor(b)) for b in vb]).to(dev)
+        Tm, lm = hotrg.renorm2d(T0, chi, 20, 20, policy)
+        zm = hotrg.lnz2d(Tm, lm, 40).cpu().numpy()
+        del Tm
+        Td, ld = hotrg.renorm2d(T0, chi, 20, 20, hotrg.ALL_FP64)
+        zd = hotrg.lnz2d(Td, ld, 40).cpu().numpy()
+        del Td
+        out["lnZ_err_vs_fp64"] = float(np.max(np.abs(zm - zd)))
+        out["lnZ_check"] = {"betas": vb, "mixed": zm.tolist(), "fp64": zd.tolist()}
+        torch.cuda.empty_cache()
+
+    # ---- CPU baseline: the NumPy float64 oracle on the host ----
+    if not args.no_cpu:
+        from oracle import HOTRG_2d as oh2
+        Tc = models.ising2d_tensor(0.44)
+        s, d = 0, "Y"
+        while Tc.shape[0] < chi or Tc.shape[2] < chi:
+            s += 1
+            Tc, _ = oh2.__tensor_normalization__(oh2.new_pure_tensor(Tc, chi, d))
+            d = "X" if d == "Y" else "Y"
+        t1 = time.perf_counter()
+        Tc, _ = oh2.__tensor_normalization__(oh2.new_pure_tensor(Tc, chi, d))
+        tc = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": 1.0 / tc, "unit": "RG steps/s", "cores": os.cpu_count(), "kind": "port",
+                               "sample": f"one steady-state chi={chi} {d} step of one instance (beta=0.44), "
+                                         f"NumPy/OpenBLAS float64 oracle, {tc:.1f} s"}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

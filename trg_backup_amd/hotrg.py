@@ -162,7 +162,8 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
         A = T1m[:, :, c0 * Dy * Dy:c1 * Dy * Dy].transpose(1, 2)          # (B, (c,k,e), a)
         Z1 = gemm(A, Uam, compute=compute)                                 # [(c,k,e),(b,i)]
         Z1p = permute(Z1.view(B, nc, Dy, Dy, Dx, chi_a), (0, 3, 4, 1, 2, 5))   # [e,b,c,k,i]
-        Z2 = gemm(Z1p.view(B, Dy * Dx, nc * Dy * chi_a).transpose(1, 2), Tp, compute=compute)  # [(c,k,i),(d,l)]
+        Z2 = gemm(Z1p.view(B, Dy * Dx, nc * Dy * chi_a).transpose(1, 2), Tp, compute=compute,
+                  tag="merge2d_z2")                                        # [(c,k,i),(d,l)]
         Z2p = permute(Z2.view(B, nc, Dy, chi_a, Dx, Dy), (0, 2, 3, 5, 1, 4))   # [k,i,l,c,d]
         gemm(Z2p.view(B, Dy * chi_a * Dy, nc * Dx), Ubm[:, c0 * Dx:c1 * Dx, :], outm,
              beta=0.0 if c0 == 0 else 1.0, compute=compute)
@@ -305,7 +306,8 @@ def merge3d(T1, T2, U1a, U1b, U2a, U2b, compute=None):
         # X2 legs (j,l,e,z,m,a,o,c) -> [(j,l,e,a,c),(z,m,o)]
         X2p = permute(X2.view(B, Dx, Dy, Dz, Dz, Dx, nca, Dy, cc_), (0, 1, 2, 3, 6, 8, 4, 5, 7))
         del X1, X1p, X2
-        X3 = gemm(X2p.view(B, Dx * Dy * Dz * nca * cc_, Dz * Dx * Dy), T2p, compute=compute)  # [(j,l,e,a,c),(n,p,f)]
+        X3 = gemm(X2p.view(B, Dx * Dy * Dz * nca * cc_, Dz * Dx * Dy), T2p, compute=compute,
+                  tag="merge3d_x3")                                        # [(j,l,e,a,c),(n,p,f)]
         del X2p
         # X3 legs (j,l,e,a,c,n,p,f) -> [(l,e,a,c,p,f),(j,n)]
         X3p = permute(X3.view(B, Dx, Dy, Dz, nca, cc_, Dx, Dy, Dz), (0, 2, 3, 4, 5, 7, 8, 1, 6))

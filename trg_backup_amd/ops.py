@@ -99,7 +99,36 @@ def fill_(t, value):
 # ---------------------------------------------------------------------------
 # GEMM
 # ---------------------------------------------------------------------------
-def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None):
+class GemmRecorder:
+    """Records HIP events around tagged GEMM launches on the launching stream
+    (used to measure a kernel's live launch duration inside a timed region)."""
+
+    def __init__(self, tag):
+        self.tag = tag
+        self.events = []   # (start, end, flops)
+
+    def __enter__(self):
+        _RECORDERS.append(self)
+        return self
+
+    def __exit__(self, *exc):
+        _RECORDERS.remove(self)
+        return False
+
+    def summary(self):
+        torch.cuda.synchronize()
+        if not self.events:
+            return {"launches": 0}
+        ms = [a.elapsed_time(b) for a, b, _ in self.events]
+        fl = [f for _, _, f in self.events]
+        return {"launches": len(ms), "avg_ms": sum(ms) / len(ms), "avg_flops": sum(fl) / len(fl),
+                "total_ms": sum(ms), "total_flops": sum(fl)}
+
+
+_RECORDERS = []
+
+
+def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None):
     """C[b] = alpha A[b] B[b] + beta C[b] for 2D or batched 3D strided views.
 
     A is (M, K) or (batch, M, K), B is (K, N) or (batch, K, N).  Views may
@@ -152,8 +181,16 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None):
         cc = compute
     if dt == F64:
         cc = F64
+    recs = [r for r in _RECORDERS if tag is not None and r.tag == tag]
+    if recs:
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
     call("trg_gemm", dt, cc, batch, M, N, K, float(alpha), ptr(A), sA, rsA, csA, ptr(B), sB, rsB, csB,
          float(beta), ptr(C), C.stride(0) if batch > 1 else 0, C.stride(1), C.stride(2), stream_ptr())
+    if recs:
+        ev1.record()
+        for r in recs:
+            r.events.append((ev0, ev1, 2.0 * M * N * K * batch))
     return C if batched else C[0]
 
 
