@@ -68,7 +68,7 @@ def main():
     # ---- growth to the steady shape (untimed) ----
     T = torch.stack([torch.as_tensor(models.ising2d_tensor(b)) for b in betas]).to(dev)
     nsteps_total = 6 + args.warmup + args.steps
-    lnf = torch.zeros((nb, nsteps_total + 1), dtype=torch.float64, device=dev)
+    lnf = ops.fill_(torch.empty((nb, nsteps_total + 1), dtype=torch.float64, device=dev), 0.0)
     T = ops.contiguous(T)
     hotrg.normalize_maxabs(T, lnf, 0)
     step = 0
@@ -132,7 +132,7 @@ def main():
     # ---- per-instance float64 path with a host sync per step ----
     if not args.no_plain:
         T1 = torch.as_tensor(models.ising2d_tensor(0.44)).to(dev)[None].contiguous()
-        l1 = torch.zeros((1, 64), dtype=torch.float64, device=dev)
+        l1 = ops.fill_(torch.empty((1, 64), dtype=torch.float64, device=dev), 0.0)
         s, d = 0, "Y"
         while T1.shape[1] < chi or T1.shape[3] < chi:
             s += 1

@@ -141,3 +141,19 @@ def test_eigh_topk(gpu, n, k):
     assert np.linalg.norm(P - P0) < 1e-9
     assert abs(eps - lam[k:].sum()) < 1e-10 * lam.sum()
     assert int(info.cpu().numpy()[0]) == 0
+
+
+@pytest.mark.parametrize("perm", [(0, 1, 2, 3), (0, 2, 1, 3), (3, 1, 2, 0), (0, 1, 3, 2), (2, 3, 0, 1), (1, 3, 0, 2)])
+def test_permute_fast_paths(gpu, perm):
+    """Row-copy and tiled-transpose paths of trg_copy_strided (odd sizes, tails, casts)."""
+    from trg_backup_amd import ops
+    x = np.random.default_rng(7).standard_normal((5, 33, 40, 17 * 3))
+    xd = _dev(x, gpu)
+    y = ops.permute(xd, perm)
+    assert np.array_equal(y.cpu().numpy(), np.transpose(x, perm))
+    y32 = ops.permute(xd, perm, dtype=torch.float32)
+    assert np.array_equal(y32.cpu().numpy(), np.transpose(x, perm).astype(np.float32))
+    # strided (sliced) input view
+    xs = xd[:, 3:30, ::2, :]
+    ys = ops.permute(xs, perm)
+    assert np.array_equal(ys.cpu().numpy(), np.transpose(x[:, 3:30, ::2, :], perm))

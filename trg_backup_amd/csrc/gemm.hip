@@ -105,7 +105,7 @@ __device__ static inline void tile_coords(const GemmArgs &g, int &tm, int &tn) {
 // fp32 / fp64 kernel.  LDS holds As[k][m] and Bs[k][n] (m / n fastest) so a
 // wave's operand fetch is one contiguous row segment per k.
 // ---------------------------------------------------------------------------
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool A_MCONT, bool B_NCONT>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool A_MCONT, bool B_NCONT, int VEC>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
 gemm_kernel(GemmArgs g) {
     using MF = Mfma<T>;
@@ -139,40 +139,73 @@ gemm_kernel(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = MF::zero();
 
+    // Global loads move VEC consecutive elements of an operand's contiguous
+    // leg at once (VEC = 4 for 16-byte fp32 loads when sizes and strides
+    // are multiples of 4, so a vector is either fully inside or outside).
+    static_assert(AE % VEC == 0 && BE % VEC == 0, "vector width must divide the per-thread tile share");
+    typedef T vec_t __attribute__((ext_vector_type(VEC)));
     T ra[AE], rb[BE];
 
+    auto a_index = [&](int i, int &mm, int &kk) {
+        const int e = tid + i * NT;
+        if (A_MCONT) { kk = e / (BM / VEC); mm = VEC * (e % (BM / VEC)); }
+        else { mm = e / (BK / VEC); kk = VEC * (e % (BK / VEC)); }
+    };
+    auto b_index = [&](int i, int &nn, int &kk) {
+        const int e = tid + i * NT;
+        if (B_NCONT) { kk = e / (BN / VEC); nn = VEC * (e % (BN / VEC)); }
+        else { nn = e / (BK / VEC); kk = VEC * (e % (BK / VEC)); }
+    };
     auto load = [&](int k0) {
 #pragma unroll
-        for (int i = 0; i < AE; ++i) {
-            const int e = tid + i * NT;
+        for (int i = 0; i < AE / VEC; ++i) {
             int mm, kk;
-            if (A_MCONT) { kk = e / BM; mm = e % BM; } else { mm = e / BK; kk = e % BK; }
+            a_index(i, mm, kk);
             const int m = m0 + mm, k = k0 + kk;
-            ra[i] = (m < g.M && k < g.K) ? A[(long long)m * g.rsA + (long long)k * g.csA] : T(0);
+            if (m < g.M && k < g.K) {
+                const vec_t v = *reinterpret_cast<const vec_t *>(A + (long long)m * g.rsA + (long long)k * g.csA);
+#pragma unroll
+                for (int t = 0; t < VEC; ++t) ra[i * VEC + t] = v[t];
+            } else {
+#pragma unroll
+                for (int t = 0; t < VEC; ++t) ra[i * VEC + t] = T(0);
+            }
         }
 #pragma unroll
-        for (int i = 0; i < BE; ++i) {
-            const int e = tid + i * NT;
+        for (int i = 0; i < BE / VEC; ++i) {
             int nn, kk;
-            if (B_NCONT) { kk = e / BN; nn = e % BN; } else { nn = e / BK; kk = e % BK; }
+            b_index(i, nn, kk);
             const int n = n0 + nn, k = k0 + kk;
-            rb[i] = (n < g.N && k < g.K) ? B[(long long)k * g.rsB + (long long)n * g.csB] : T(0);
+            if (n < g.N && k < g.K) {
+                const vec_t v = *reinterpret_cast<const vec_t *>(B + (long long)k * g.rsB + (long long)n * g.csB);
+#pragma unroll
+                for (int t = 0; t < VEC; ++t) rb[i * VEC + t] = v[t];
+            } else {
+#pragma unroll
+                for (int t = 0; t < VEC; ++t) rb[i * VEC + t] = T(0);
+            }
         }
     };
     auto store = [&]() {
 #pragma unroll
-        for (int i = 0; i < AE; ++i) {
-            const int e = tid + i * NT;
+        for (int i = 0; i < AE / VEC; ++i) {
             int mm, kk;
-            if (A_MCONT) { kk = e / BM; mm = e % BM; } else { mm = e / BK; kk = e % BK; }
-            As[kk][mm] = ra[i];
+            a_index(i, mm, kk);
+#pragma unroll
+            for (int t = 0; t < VEC; ++t) {
+                if (A_MCONT) As[kk][mm + t] = ra[i * VEC + t];
+                else As[kk + t][mm] = ra[i * VEC + t];
+            }
         }
 #pragma unroll
-        for (int i = 0; i < BE; ++i) {
-            const int e = tid + i * NT;
+        for (int i = 0; i < BE / VEC; ++i) {
             int nn, kk;
-            if (B_NCONT) { kk = e / BN; nn = e % BN; } else { nn = e / BK; kk = e % BK; }
-            Bs[kk][nn] = rb[i];
+            b_index(i, nn, kk);
+#pragma unroll
+            for (int t = 0; t < VEC; ++t) {
+                if (B_NCONT) Bs[kk][nn + t] = rb[i * VEC + t];
+                else Bs[kk + t][nn] = rb[i * VEC + t];
+            }
         }
     };
 
@@ -345,17 +378,35 @@ gemm_bf16_kernel(GemmArgs g) {
 // ---------------------------------------------------------------------------
 // Host dispatch.
 // ---------------------------------------------------------------------------
-template <typename T, int BM, int BN, int BK, int WM, int WN>
-static int launch_t(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
+template <typename T, int BM, int BN, int BK, int WM, int WN, int VEC>
+static int launch_v(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
     g.tiles_m = (int)cdiv(g.M, BM);
     g.tiles_n = (int)cdiv(g.N, BN);
     dim3 grid(g.tiles_m * g.tiles_n, batch);
     dim3 block((BM / WM) * (BN / WN) * 64);
-    if (amc && bnc) hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, true>), grid, block, 0, s, g);
-    else if (amc) hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, false>), grid, block, 0, s, g);
-    else if (bnc) hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, true>), grid, block, 0, s, g);
-    else hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, false>), grid, block, 0, s, g);
+    if (amc && bnc) hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, true, VEC>), grid, block, 0, s, g);
+    else if (amc) hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, true, false, VEC>), grid, block, 0, s, g);
+    else if (bnc) hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, true, VEC>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, false, false, VEC>), grid, block, 0, s, g);
     return check_launch();
+}
+
+// 16-byte vector loads are legal when every vector starts on a 16-byte
+// boundary and never straddles the operand's edge.
+static bool vec_ok(const GemmArgs &g, bool amc, bool bnc, int esize, int vec) {
+    auto al = [&](const void *p, long long sb) {
+        return (reinterpret_cast<uintptr_t>(p) % (esize * vec) == 0) && (sb % vec == 0);
+    };
+    const bool a = al(g.A, g.sAb) && (amc ? (g.M % vec == 0 && g.csA % vec == 0) : (g.K % vec == 0 && g.rsA % vec == 0));
+    const bool b = al(g.B, g.sBb) && (bnc ? (g.N % vec == 0 && g.rsB % vec == 0) : (g.K % vec == 0 && g.csB % vec == 0));
+    return a && b;
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+static int launch_t(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
+    constexpr int V = 16 / (int)sizeof(T);
+    if (vec_ok(g, amc, bnc, sizeof(T), V)) return launch_v<T, BM, BN, BK, WM, WN, V>(g, batch, amc, bnc, s);
+    return launch_v<T, BM, BN, BK, WM, WN, 1>(g, batch, amc, bnc, s);
 }
 
 template <int BM, int BN, int BK, int WM, int WN>
@@ -395,7 +446,9 @@ extern "C" int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, 
     GemmArgs g{M, N, K, A, sAb, rsA, csA, B, sBb, rsB, csB, C, sCb, rsC, csC, alpha, beta, 0, 0};
     const long long work = (long long)M * N;
     if (dtype == TRG_F32 && compute == TRG_F32) {
-        if (work >= 256LL * 128 * 128) return launch_t<float, 128, 128, 16, 64, 64>(g, batch, amc, bnc, s);
+        if (N <= 64 && M >= 4096) return launch_t<float, 256, 64, 16, 64, 64>(g, batch, amc, bnc, s);
+        if (M <= 64 && N >= 4096) return launch_t<float, 64, 256, 16, 64, 64>(g, batch, amc, bnc, s);
+        if (work >= 256LL * 128 * 128) return launch_t<float, 128, 128, 32, 64, 64>(g, batch, amc, bnc, s);
         return launch_t<float, 64, 64, 16, 32, 32>(g, batch, amc, bnc, s);
     }
     if (dtype == TRG_F32 && compute == TRG_BF16) {

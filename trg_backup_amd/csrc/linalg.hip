@@ -16,6 +16,7 @@
 #include <math.h>
 
 #include <mutex>
+#include <utility>
 
 #include "common.h"
 
@@ -232,18 +233,25 @@ __global__ void __launch_bounds__(256) chol_inv_kernel(int l, double *Gg, long l
         }
         __syncthreads();
     }
-    // X = R^{-1}: back substitution, one column per thread
-    for (int c = tid; c < l; c += 256) {
-        for (int i = l - 1; i >= 0; --i) {
+    // In-place inverse of the upper factor, column by column: with the
+    // leading j x j block already inverted (X), column j of X is
+    // -X[0:j,0:j] R[0:j,j] / R[j][j] and X[j][j] = 1 / R[j][j].
+    __shared__ double colv[256];
+    for (int j = 0; j < l; ++j) {
+        const double djj = 1.0 / G[j * l + j];
+        for (int i = tid; i < j; i += 256) {
             double v = 0.0;
-            if (i > c) {
-                X[i * l + c] = 0.0;
-                continue;
-            }
-            if (i == c) v = 1.0;
-            for (int k = i + 1; k <= c; ++k) v -= G[i * l + k] * X[k * l + c];
-            X[i * l + c] = v / G[i * l + i];
+            for (int k = i; k < j; ++k) v += G[i * l + k] * G[k * l + j];
+            colv[i] = -v * djj;
         }
+        __syncthreads();
+        for (int i = tid; i < j; i += 256) G[i * l + j] = colv[i];
+        if (tid == 0) G[j * l + j] = djj;
+        __syncthreads();
+    }
+    for (int e = tid; e < l * l; e += 256) {
+        const int i = e / l, c = e % l;
+        X[e] = (i <= c) ? G[e] : 0.0;
     }
     if (tid == 0 && info) info[b] = bad;
 }
@@ -334,7 +342,7 @@ extern "C" long long trg_eigh_topk_workspace(int batch, int n, int l) {
 
 // Orthonormalise X (batch of n x l) with shifted CholeskyQR3 (one shifted
 // pass, two plain passes; Fukaya et al., SIAM J. Sci. Comput. 42 (2020)).
-// Y is scratch of the same size; the result is left in X.
+// After the three passes the orthonormal result is in Y (X is clobbered).
 static int cholqr3(int batch, int n, int l, double *X, double *Y, double *G, double *R, int *flags, int *info,
                    hipStream_t s) {
     const double shift0 = 11.0 * ((double)n * l + (double)l * (l + 1)) * 1.1102230246251565e-16;
@@ -352,10 +360,7 @@ static int cholqr3(int batch, int n, int l, double *X, double *Y, double *G, dou
         TRG_TRY(trg_gemm(TRG_F64, TRG_F64, batch, n, l, l, 1.0, src, sX, l, 1, R, sG, l, 1, 0.0, dst, sX, l, 1, s));
         double *t = src; src = dst; dst = t;
     }
-    // an odd number of passes leaves the result in Y
-    long long dims[1] = {(long long)batch * n * l};
-    long long st[1] = {1};
-    return trg_copy_strided(TRG_F64, TRG_F64, 1, dims, src, st, X, st, s);
+    return TRG_OK;
 }
 
 extern "C" int trg_eigh_topk_f64(int batch, int n, int k, int l, int q, const double *A, long long sA, double *w,
@@ -385,6 +390,7 @@ extern "C" int trg_eigh_topk_f64(int batch, int n, int k, int l, int q, const do
     // Gaussian start block, the same draw for every instance (batch-invariant)
     TRG_TRY(trg_gaussian(TRG_F64, batch, sX, seed, 0u, 0u, 0ull, Q, sX, s));
     TRG_TRY(cholqr3(batch, n, l, Q, Y, G, R, flags, info, s));
+    std::swap(Q, Y);
     // (A + sigma I) keeps every iterate of full column rank; sigma is far
     // below the kept eigenvalues and the Ritz step uses A itself.
     const double shift_rel = 1e-13;
@@ -393,9 +399,7 @@ extern "C" int trg_eigh_topk_f64(int batch, int n, int k, int l, int q, const do
         const dim3 grid((unsigned)std::min<long long>(cdiv(sX, 256), 4096), batch);
         hipLaunchKernelGGL(axpy_scaled_kernel, grid, 256, 0, s, sX, shift_rel, tr, Q, sX, Y, sX);
         TRG_TRY(check_launch());
-        TRG_TRY(cholqr3(batch, n, l, Y, Q, G, R, flags, info, s));
-        long long dims[1] = {(long long)batch * sX}, st[1] = {1};
-        TRG_TRY(trg_copy_strided(TRG_F64, TRG_F64, 1, dims, Y, st, Q, st, s));
+        TRG_TRY(cholqr3(batch, n, l, Y, Q, G, R, flags, info, s));   // result lands in Q
     }
     // Rayleigh-Ritz: H = Q^T A Q, H = W diag(theta) W^T
     TRG_TRY(trg_gemm(TRG_F64, TRG_F64, batch, n, l, n, 1.0, A, sA, n, 1, Q, sX, l, 1, 0.0, Y, sX, l, 1, s));

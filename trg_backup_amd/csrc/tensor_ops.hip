@@ -35,6 +35,99 @@ __global__ void copy_strided_kernel(StridedDesc d, const Ti *__restrict__ in, To
     }
 }
 
+// Tiled transpose-copy for a permutation whose input-contiguous leg (ii)
+// differs from the output-contiguous leg (oi): a 32 x 32 tile is read
+// coalesced along ii, staged in LDS and written coalesced along oi.  The
+// remaining legs are enumerated by the workgroup index (decomposed once per
+// workgroup).  Output and input strides are arbitrary.
+constexpr int TT = 32;
+struct TransDesc {
+    int nouter;
+    long long odims[6];
+    long long osi[6], oso[6];  // strides of the outer legs
+    long long ni, nj;          // extents of leg ii (input-contiguous) and oi (output-contiguous)
+    long long si_j, so_i;      // input stride of oi, output stride of ii
+    long long tiles_i, tiles_j;
+    long long nblk;            // number of tiles in the launch
+};
+
+template <typename Ti, typename To>
+__global__ void __launch_bounds__(256) transpose_kernel(TransDesc d, const Ti *__restrict__ in, To *__restrict__ out) {
+    __shared__ To tile[TT][TT + 1];
+    long long bid = blockIdx.x + (long long)blockIdx.y * gridDim.x;
+    if (bid >= d.nblk) return;
+    const long long tiles = d.tiles_i * d.tiles_j;
+    long long outer = bid / tiles;
+    const long long t = bid - outer * tiles;
+    const long long ti = t % d.tiles_i, tj = t / d.tiles_i;
+    long long offi = 0, offo = 0;
+    for (int k = d.nouter - 1; k >= 0; --k) {
+        const long long ik = outer % d.odims[k];
+        outer /= d.odims[k];
+        offi += ik * d.osi[k];
+        offo += ik * d.oso[k];
+    }
+    const int tx = threadIdx.x & (TT - 1), ty = threadIdx.x / TT;   // 32 x 8
+    const long long i0 = ti * TT, j0 = tj * TT;
+#pragma unroll
+    for (int r = 0; r < TT; r += 8) {
+        const long long i = i0 + tx, j = j0 + ty + r;
+        if (i < d.ni && j < d.nj) tile[ty + r][tx] = (To)in[offi + i + j * d.si_j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < TT; r += 8) {
+        const long long j = j0 + tx, i = i0 + ty + r;
+        if (i < d.ni && j < d.nj) out[offo + j + i * d.so_i] = tile[tx][ty + r];
+    }
+}
+
+// Row copy when input and output share their contiguous leg: each
+// workgroup copies ROWS rows of length n (coalesced on both sides).
+struct RowDesc {
+    int nouter;
+    long long odims[7];
+    long long osi[7], oso[7];
+    long long n;        // row length
+    long long rows;
+};
+
+// Each workgroup copies a tile of RCH elements: either R = RCH / n whole rows
+// (short rows, every thread busy) or one RCH-long segment of a long row.
+// Row base offsets are computed once per row into LDS.
+constexpr int RCH = 4096;
+
+template <typename Ti, typename To>
+__global__ void __launch_bounds__(256) rowcopy_kernel(RowDesc d, const Ti *__restrict__ in, To *__restrict__ out,
+                                                      long long nblk) {
+    __shared__ long long offi_s[256], offo_s[256];
+    const long long bid = blockIdx.x + (long long)blockIdx.y * gridDim.x;
+    if (bid >= nblk) return;
+    const long long R = d.n >= RCH ? 1 : min(256LL, RCH / d.n);   // rows in this tile
+    const long long segs = d.n >= RCH ? (d.n + RCH - 1) / RCH : 1;  // segments per row
+    const long long r0 = (bid / segs) * R;
+    const long long c0 = (bid % segs) * RCH;
+    const long long ncol = min((long long)RCH, d.n - c0);
+    const int nr = (int)min(R, d.rows - r0);
+    for (int rr = threadIdx.x; rr < nr; rr += blockDim.x) {
+        long long o = r0 + rr, oi = 0, oo = 0;
+        for (int k = d.nouter - 1; k >= 0; --k) {
+            const long long ik = o % d.odims[k];
+            o /= d.odims[k];
+            oi += ik * d.osi[k];
+            oo += ik * d.oso[k];
+        }
+        offi_s[rr] = oi + c0;
+        offo_s[rr] = oo + c0;
+    }
+    __syncthreads();
+    const int tot = (int)(nr * ncol);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int rr = e / (int)ncol, c = e - rr * (int)ncol;
+        out[offo_s[rr] + c] = (To)in[offi_s[rr] + c];
+    }
+}
+
 template <typename T>
 __global__ void fill_kernel(T *out, long long n, T v) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -246,6 +339,63 @@ extern "C" int trg_copy_strided(int dtype_in, int dtype_out, int rank, const lon
     if (total == 0) return TRG_OK;
     if (!in || !out) return TRG_BAD_ARG;
     hipStream_t s = as_stream(stream);
+    {
+        // fast paths: the leg contiguous in the output (oi) and in the input (ii)
+        int oi = -1, ii = -1;
+        for (int k = 0; k < rank; ++k) {
+            if (so[k] == 1 && dims[k] > 1) oi = k;
+            if (si[k] == 1 && dims[k] > 1) ii = k;
+        }
+        if (oi >= 0 && ii >= 0 && oi == ii && dims[oi] >= 16) {
+            RowDesc r;
+            r.nouter = 0;
+            for (int k = 0; k < rank; ++k) {
+                if (k == oi) continue;
+                r.odims[r.nouter] = dims[k];
+                r.osi[r.nouter] = si[k];
+                r.oso[r.nouter] = so[k];
+                ++r.nouter;
+            }
+            r.n = dims[oi];
+            r.rows = total / dims[oi];
+            const long long R = r.n >= RCH ? 1 : std::min(256LL, RCH / r.n);
+            const long long segs = r.n >= RCH ? cdiv(r.n, RCH) : 1;
+            const long long nblk = cdiv(r.rows, R) * segs;
+            const dim3 grid((unsigned)std::min<long long>(nblk, 65535), (unsigned)cdiv(nblk, 65535));
+            if (dtype_in == TRG_F32 && dtype_out == TRG_F32) hipLaunchKernelGGL((rowcopy_kernel<float, float>), grid, 256, 0, s, r, (const float *)in, (float *)out, nblk);
+            else if (dtype_in == TRG_F64 && dtype_out == TRG_F64) hipLaunchKernelGGL((rowcopy_kernel<double, double>), grid, 256, 0, s, r, (const double *)in, (double *)out, nblk);
+            else if (dtype_in == TRG_F32 && dtype_out == TRG_F64) hipLaunchKernelGGL((rowcopy_kernel<float, double>), grid, 256, 0, s, r, (const float *)in, (double *)out, nblk);
+            else if (dtype_in == TRG_F64 && dtype_out == TRG_F32) hipLaunchKernelGGL((rowcopy_kernel<double, float>), grid, 256, 0, s, r, (const double *)in, (float *)out, nblk);
+            else return TRG_UNSUPPORTED;
+            return check_launch();
+        }
+        if (oi >= 0 && ii >= 0 && oi != ii && dims[oi] >= 8 && dims[ii] >= 8 && rank <= 8) {
+            TransDesc t;
+            t.nouter = 0;
+            for (int k = 0; k < rank; ++k) {
+                if (k == oi || k == ii) continue;
+                t.odims[t.nouter] = dims[k];
+                t.osi[t.nouter] = si[k];
+                t.oso[t.nouter] = so[k];
+                ++t.nouter;
+            }
+            t.ni = dims[ii];
+            t.nj = dims[oi];
+            t.si_j = si[oi];
+            t.so_i = so[ii];
+            t.tiles_i = cdiv(t.ni, TT);
+            t.tiles_j = cdiv(t.nj, TT);
+            const long long nblk = t.tiles_i * t.tiles_j * (total / (t.ni * t.nj));
+            t.nblk = nblk;
+            const dim3 grid((unsigned)std::min<long long>(nblk, 65535), (unsigned)cdiv(nblk, 65535));
+            if (dtype_in == TRG_F32 && dtype_out == TRG_F32) hipLaunchKernelGGL((transpose_kernel<float, float>), grid, 256, 0, s, t, (const float *)in, (float *)out);
+            else if (dtype_in == TRG_F64 && dtype_out == TRG_F64) hipLaunchKernelGGL((transpose_kernel<double, double>), grid, 256, 0, s, t, (const double *)in, (double *)out);
+            else if (dtype_in == TRG_F32 && dtype_out == TRG_F64) hipLaunchKernelGGL((transpose_kernel<float, double>), grid, 256, 0, s, t, (const float *)in, (double *)out);
+            else if (dtype_in == TRG_F64 && dtype_out == TRG_F32) hipLaunchKernelGGL((transpose_kernel<double, float>), grid, 256, 0, s, t, (const double *)in, (float *)out);
+            else return TRG_UNSUPPORTED;
+            return check_launch();
+        }
+    }
     const unsigned g = grid_for(total, 256);
     if (dtype_in == TRG_F32 && dtype_out == TRG_F32)
         hipLaunchKernelGGL((copy_strided_kernel<float, float>), g, 256, 0, s, d, (const float *)in, (float *)out, total);

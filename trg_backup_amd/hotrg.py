@@ -80,20 +80,26 @@ ALL_FP64 = Policy()
 # ---------------------------------------------------------------------------
 # isometries
 # ---------------------------------------------------------------------------
-def topk_isometry(envs, k):
-    """envs: list of float64 (B, n, n) environments.  Returns per env
-    (V (B, n, k) eigenvectors of the k largest eigenvalues, ascending, and
-    eps (B,) = discarded eigenvalue mass)."""
-    out = []
-    for MM in envs:
-        B, n, _ = MM.shape
-        l = min(n, max(EIG["l_factor"] * k, k + 8), 256)
-        if k > l:
-            raise TrgError(f"bond dimension {k} above the eigensolver's block limit")
-        q = 0 if l == n else EIG["q"]
-        _, V, eps, _ = ops.eigh_topk(MM, k, l, q, seed=EIG["seed"], batch=B)
-        out.append((V, eps))
-    return out
+def topk_isometry(MM, k):
+    """MM: float64 (nb, n, n) environments (every instance and side stacked
+    in one batch).  Returns V (nb, n, k), the eigenvectors of the k largest
+    eigenvalues in ascending order, and eps (nb,), the discarded eigenvalue
+    mass tr(MM) - sum of the kept eigenvalues."""
+    nb, n, _ = MM.shape
+    l = min(n, max(EIG["l_factor"] * k, k + 8), 256)
+    if k > l:
+        raise TrgError(f"bond dimension {k} above the eigensolver's block limit")
+    q = 0 if l == n else EIG["q"]
+    _, V, eps, _ = ops.eigh_topk(MM, k, l, q, seed=EIG["seed"], batch=nb)
+    return V, eps
+
+
+def _pick(MM, B, Dcut, dtype):
+    """Eigen-decompose both sides' environments (MM[:B] = side 0, MM[B:] =
+    side 1) in one batched call and keep the side with the smaller
+    discarded mass, on the device."""
+    V, eps = topk_isometry(MM, Dcut)
+    return ops.select_isometry(eps[:B], eps[B:], V[:B], V[B:], dtype, batch=B)
 
 
 def _identity_isometry(B, D, dtype, device):
@@ -104,8 +110,9 @@ def _identity_isometry(B, D, dtype, device):
 # ---------------------------------------------------------------------------
 # 2D
 # ---------------------------------------------------------------------------
-def env2d(T, leg, compute=None):
-    """MM_{(ab),(cd)} of trg/HOTRG_2d.py:26 (leg 0) / :27 (leg 1) as float64 (B, Dx^2, Dx^2).
+def env2d(T, leg, compute=None, out=None):
+    """MM_{(ab),(cd)} of trg/HOTRG_2d.py:26 (leg 0) / :27 (leg 1) as float64 (B, Dx^2, Dx^2)
+    (written into `out` when given).
 
     A[a,k,c,n] = sum_ij T[a,i,j,k] T[c,i,j,n]; B[b,k,d,n] = sum_lm T[b,l,k,m] T[d,l,n,m];
     MM[(a,b),(c,d)] = sum_kn A[a,k,c,n] B[b,k,d,n].
@@ -119,7 +126,11 @@ def env2d(T, leg, compute=None):
     A1p = permute(A1.view(B, Dx, Dy, Dx, Dy), (0, 1, 3, 2, 4)).view(B, Dx * Dx, Dy * Dy)  # [(a,c),(k,n)]
     B1p = permute(B1.view(B, Dx, Dy, Dx, Dy), (0, 1, 3, 2, 4)).view(B, Dx * Dx, Dy * Dy)  # [(b,d),(k,n)]
     M = gemm(A1p, B1p.transpose(1, 2), compute=compute)                   # [(a,c),(b,d)]
-    return permute(M.view(B, Dx, Dx, Dx, Dx), (0, 1, 3, 2, 4), dtype=torch.float64).view(B, Dx * Dx, Dx * Dx)
+    src = M.view(B, Dx, Dx, Dx, Dx).permute(0, 1, 3, 2, 4)
+    if out is None:
+        out = torch.empty((B, Dx * Dx, Dx * Dx), dtype=torch.float64, device=T.device)
+    ops.copy_into(out.view(B, Dx, Dx, Dx, Dx), src)
+    return out
 
 
 def projector2d(T, Dcut, prec=FP64):
@@ -131,8 +142,10 @@ def projector2d(T, Dcut, prec=FP64):
         ops.fill_(flag, 0)
         return _identity_isometry(B, Dx, T.dtype, T.device), flag
     c = prec.env_compute()
-    (V1, e1), (V2, e2) = topk_isometry([env2d(T, 0, c), env2d(T, 1, c)], Dcut)
-    U, flag = ops.select_isometry(e1, e2, V1, V2, T.dtype, batch=B)
+    MM = torch.empty((2 * B, n, n), dtype=torch.float64, device=T.device)
+    env2d(T, 0, c, out=MM[:B])
+    env2d(T, 1, c, out=MM[B:])
+    U, flag = _pick(MM, B, Dcut, T.dtype)
     return U.view(B, Dx, Dx, Dcut), flag
 
 
@@ -230,7 +243,7 @@ def lnz2d(T, lnf, nsteps):
 # ---------------------------------------------------------------------------
 # 3D
 # ---------------------------------------------------------------------------
-def env3d(T, leg, compute=None):
+def env3d(T, leg, compute=None, out=None):
     """MM_dag1m/1p/2m/2p of trg/HOTRG_3d.py:42-45 for leg 0..3 as float64 (B, D^2, D^2).
 
     With T1 = T2 = T: A[a,m,c,n] = Gram of T over its 4 legs other than
@@ -252,7 +265,10 @@ def env3d(T, leg, compute=None):
     A1p = permute(A1.view(Bt, D, Dz, D, Dz), (0, 1, 3, 2, 4)).view(Bt, D * D, Dz * Dz)
     B1p = permute(B1.view(Bt, D, Dz, D, Dz), (0, 1, 3, 2, 4)).view(Bt, D * D, Dz * Dz)
     M = gemm(A1p, B1p.transpose(1, 2), compute=compute)                 # [(a,c),(b,d)]
-    return permute(M.view(Bt, D, D, D, D), (0, 1, 3, 2, 4), dtype=torch.float64).view(Bt, D * D, D * D)
+    if out is None:
+        out = torch.empty((Bt, D * D, D * D), dtype=torch.float64, device=T.device)
+    ops.copy_into(out.view(Bt, D, D, D, D), M.view(Bt, D, D, D, D).permute(0, 1, 3, 2, 4))
+    return out
 
 
 def projector3d(T, Dcut, prec=FP64):
@@ -267,8 +283,10 @@ def projector3d(T, Dcut, prec=FP64):
             ops.fill_(flag, 0)
             res.append((_identity_isometry(B, D, T.dtype, T.device), flag))
             continue
-        (Vm, em), (Vp, ep) = topk_isometry([env3d(T, legs[0], c), env3d(T, legs[1], c)], Dcut)
-        U, flag = ops.select_isometry(em, ep, Vm, Vp, T.dtype, batch=B)
+        MM = torch.empty((2 * B, n, n), dtype=torch.float64, device=T.device)
+        env3d(T, legs[0], c, out=MM[:B])
+        env3d(T, legs[1], c, out=MM[B:])
+        U, flag = _pick(MM, B, Dcut, T.dtype)
         res.append((U.view(B, D, D, Dcut), flag))
     return res[0][0], res[1][0], res[0][1], res[1][1]
 

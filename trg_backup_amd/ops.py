@@ -260,7 +260,7 @@ def syevj(A, batch=1, max_sweeps=40):
     n = A.shape[-1]
     w = torch.empty((batch, n), dtype=torch.float64, device=A.device)
     V = torch.empty((batch, n, n), dtype=torch.float64, device=A.device)
-    info = torch.zeros(batch, dtype=torch.int32, device=A.device)
+    info = torch.empty(batch, dtype=torch.int32, device=A.device)
     call("trg_syevj_f64", batch, n, ptr(A), n * n, ptr(w), ptr(V), max_sweeps, ptr(info), stream_ptr())
     return w, V, info
 
@@ -269,7 +269,7 @@ def chol_inv(G, shift_rel=0.0, batch=1):
     require_device(G)
     l = G.shape[-1]
     R = torch.empty((batch, l, l), dtype=torch.float64, device=G.device)
-    info = torch.zeros(batch, dtype=torch.int32, device=G.device)
+    info = torch.empty(batch, dtype=torch.int32, device=G.device)
     call("trg_chol_inv_f64", batch, l, ptr(G), l * l, float(shift_rel), ptr(R), ptr(info), stream_ptr())
     return R, info
 
@@ -293,7 +293,7 @@ def eigh_topk(A, k, l, q, seed=20240, batch=1):
     w = torch.empty((batch, k), dtype=torch.float64, device=A.device)
     V = torch.empty((batch, n, k), dtype=torch.float64, device=A.device)
     eps = torch.empty(batch, dtype=torch.float64, device=A.device)
-    info = torch.zeros(batch, dtype=torch.int32, device=A.device)
+    info = torch.empty(batch, dtype=torch.int32, device=A.device)
     call("trg_eigh_topk_f64", batch, n, k, l, q, ptr(A), n * n, ptr(w), ptr(V), ptr(eps), int(seed), ptr(ws),
          ws.numel() * 8, ptr(info), stream_ptr())
     return w, V, eps, info
