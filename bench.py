@@ -123,7 +123,8 @@ def main():
                    "chi": chi, "nbeta": nb, "policy": policy.describe(), "parallelism": "single GPU, batched"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "gemm_kernel<float,128,128,16,64,64> (merge chi^7 GEMM)",
+                     "kernel": "trg::gemm_kernel<float,128,128,32,64,64,true,true,4> (merge chi^7 GEMM, "
+                               "M=nc*chi^2 N=chi^2 K=chi^2 batch=nbeta)",
                      "launches": g["launches"], "avg_ms": g["avg_ms"], "flops_per_launch": g["avg_flops"]},
     }
     del T

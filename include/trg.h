@@ -161,6 +161,51 @@ int trg_scale_rows(int dtype, int batch, long long n, long long period, long lon
                    void *Y, long long sY, const void *x, long long sx, int mode, void *stream);
 
 /*
+ * Batched one-sided Jacobi SVD of m x n float64 matrices (n <= m <= 256,
+ * m*n*8 + 16 KiB within the LDS of one workgroup).  A (stride sA,
+ * row-major) is read only; s: singular values descending (stride n), U:
+ * m x n (stride m*n), V: n x n (stride n*n, right singular vectors as
+ * columns).  Replaces cupy.linalg.svd(full_matrices=False) of the small
+ * factors in utility/truncated_svd.py:119, trg/ATRG_3d_new.py:124,416-427.
+ */
+int trg_gesvj_f64(int batch, int m, int n, const double *A, long long sA, double *s, double *U,
+                  double *V, int max_sweeps, int *info, void *stream);
+
+/*
+ * Rank selection of utility/truncated_svd.py:17-78 (`truncate`) on a
+ * descending vector s of length n: writes chi (int) and the tail-norm
+ * error (double) to device memory.
+ */
+int trg_truncate_rank(int n, const double *s, int k, double degeneracy_eps, double truncate_eps,
+                      int *chi, double *err, void *stream);
+
+/*
+ * Positive-semidefinite Cholesky of l x l blocks (l <= 256): R^T R = G with
+ * rows of R zeroed where the pivot falls below psd_tol * scale[b] (scale
+ * NULL: trace of G); R (optional) and a generalised inverse Rinv are
+ * written row-major (stride l*l).  G is overwritten.  The diagonal-block
+ * kernel of the blocked square roots R^T R = XdagX, Rx Rx^T = xxdag that
+ * replace the full SVDs of trg/ATRG_3d_new.py:271-279 (any square root
+ * gives the same squeezer).
+ */
+int trg_potrf_psd_f64(int batch, int l, double *G, long long sG, double psd_tol, const double *scale,
+                      double *R, double *Rinv, void *stream);
+
+/*
+ * Orthonormal basis of the column span of batch m x l float64 matrices
+ * (l <= 256 <= ... l <= m) by shifted CholeskyQR3; X (stride sX) is
+ * clobbered, Q is written with stride m*l.  Replaces cupy.linalg.qr in the
+ * range finders (trg/ATRG_3d_new.py:104,111,117, utility/randomized_svd.py:59,84-91).
+ */
+long long trg_orthonormalize_workspace(int batch, int m, int l);
+int trg_orthonormalize_f64(int batch, int m, int l, double *X, long long sX, double *Q, void *ws,
+                           long long ws_bytes, int *info, void *stream);
+
+/* Y[b][i] += alpha_rel * scal[b] * X[b][i] (float64, device-side scale). */
+int trg_axpy_dev_f64(int batch, long long n, double alpha_rel, const double *scal, const double *X,
+                     long long sX, double *Y, long long sY, void *stream);
+
+/*
  * ln Z / V assembly on the device: out[b] = sum_{i<nterms} lnf[b*lnf_stride+i]
  * + ln|tr[b]| * inv_volume (tr may be NULL).  Replaces the callers'
  * cp.sum(ln_normfact) + cp.log(trace)/V (measurement/Ising_model_3d.py:137,

@@ -39,9 +39,15 @@ SIGNATURES = {
     "trg_select_isometry": [c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "trg_scale_rows": [c_int, c_int, c_ll, c_ll, c_ll, c_vp, c_ll, c_vp, c_ll, c_int, c_vp],
     "trg_lnz": [c_int, c_int, c_vp, c_ll, c_vp, c_dbl, c_vp, c_vp],
+    "trg_gesvj_f64": [c_int, c_int, c_int, c_vp, c_ll, c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
+    "trg_truncate_rank": [c_int, c_vp, c_int, c_dbl, c_dbl, c_vp, c_vp, c_vp],
+    "trg_potrf_psd_f64": [c_int, c_int, c_vp, c_ll, c_dbl, c_vp, c_vp, c_vp, c_vp],
+    "trg_orthonormalize_workspace": [c_int, c_int, c_int],
+    "trg_orthonormalize_f64": [c_int, c_int, c_int, c_vp, c_ll, c_vp, c_vp, c_ll, c_vp, c_vp],
+    "trg_axpy_dev_f64": [c_int, c_ll, c_dbl, c_vp, c_vp, c_ll, c_vp, c_ll, c_vp],
 }
 _RESTYPES = {"trg_version": ctypes.c_char_p, "trg_status_string": ctypes.c_char_p,
-             "trg_eigh_topk_workspace": c_ll}
+             "trg_eigh_topk_workspace": c_ll, "trg_orthonormalize_workspace": c_ll}
 
 
 class TrgError(RuntimeError):

@@ -1,0 +1,301 @@
+"""ATRG 3D coarse graining on libtrg (trg/ATRG_3d_new.py:18-482, 954-1039).
+
+Tensor: T_{TXYtxy} = U_{TXYi} s_i VH_{itxy}; U, s, VH are GPU tensors.
+Contractions run in the configured storage dtype (float64 or float32 on
+MFMA); every decomposition is float64.
+
+Device-specific choices (documented; results are the same function of the
+tensor up to the randomized sketch and rounding):
+  * leg swapping works in the (x1, y1) / (X0, Y0) leg bases directly.  The
+    reference's SVDs of B and C (:175-188) only rotate those legs (their
+    factors UB, VCH are multiplied back at :232-233), so M = B~ C~ and its
+    singular triplets are unchanged; the rank bound chi_k is still computed
+    from the reference's reduced sizes.
+  * range-finder blocks are orthonormalised with shifted CholeskyQR3 in
+    place of Householder QR (same column span).
+  * the squeezer's environment square roots R^T R = XdagX, Rx Rx^T = xxdag
+    are positive-semidefinite Cholesky factors instead of sqrt(s) vh from
+    full SVDs (:271-279); the oblique projectors PX Px depend only on the
+    environments, not on which square root is used.
+"""
+import math
+
+import torch
+
+from . import linalg, ops
+from ._lib import TrgError
+from .contract import contract
+
+degeneracy_eps = 0.0
+truncate_eps = 1e-10
+# leg-swap rSVD parameters (reference: k_factor 3, p 0, q = Dcut, :193)
+RSVD = {"k_factor": 3, "p": 0, "q": None, "seed": 12345}
+
+
+def set_rsvd_params(k_factor=3, p=0, q=None, seed=12345):
+    RSVD.update(k_factor=k_factor, p=p, q=q, seed=seed)
+
+
+def _dt(prec):
+    return torch.float64 if prec in (None, "f64") else torch.float32
+
+
+# ---------------------------------------------------------------------------
+# leg transposition (:18-65)
+# ---------------------------------------------------------------------------
+def leg_transposition(U, VH, do_what="transpose", direction="t"):
+    if direction in ("t", "T"):
+        return U, VH
+    fwd = do_what == "transpose"
+    if direction in ("x", "X"):
+        ua, va = ((1, 2, 0, 3), (0, 2, 3, 1)) if fwd else ((2, 0, 1, 3), (0, 3, 1, 2))
+    else:
+        ua, va = ((2, 0, 1, 3), (0, 3, 1, 2)) if fwd else ((1, 2, 0, 3), (0, 2, 3, 1))
+    return ops.permute(U, ua), ops.permute(VH, va)
+
+
+# ---------------------------------------------------------------------------
+# leg swapping (:67-235)
+# ---------------------------------------------------------------------------
+def rsvd_for_ATRG_leg_swapping(Bt, Ct, chi_k, p, q, eps, seed):
+    """Randomized SVD of M_{(i,g),(j,b)} = sum_t Bt_{b,i,t} Ct_{t,j,g}
+    (:67-140).  Returns UM (i,g,k), sM (k,), VMH (k,j,b) as float64."""
+    nb, ni, nt = Bt.shape
+    _, nj, ng = Ct.shape
+    l = chi_k + p
+    rows_i, rows_j = ni * ng, nj * nb
+    ls = min(l, rows_i, rows_j)
+    dt = Bt.dtype
+
+    def apply_M(Yjbl):            # (j,b,l) -> (i,g,l)
+        W = contract("tjg,jbl->tgbl", Ct, Yjbl)
+        return contract("bit,tgbl->igl", Bt, W)
+
+    def apply_MT(Qigl):           # (i,g,l) -> (j,b,l)
+        W = contract("bit,igl->btgl", Bt, Qigl)
+        return contract("tjg,btgl->jbl", Ct, W)
+
+    if ls == rows_i or ls == rows_j or ls > linalg.SMALL_MAX:
+        if max(rows_i, rows_j) > 256 or min(rows_i, rows_j) > linalg.SMALL_MAX:
+            raise TrgError(f"leg-swap rSVD block {ls} beyond the device limits ({rows_i}x{rows_j})")
+        # the sketch spans the whole space: dense SVD of M (small early steps)
+        M = contract("bit,tjg->igjb", Bt, Ct).reshape(rows_i, rows_j)
+        u, s, vh = linalg.svd_thin(M[None])
+        UM, sM, VMH = u[0], s[0], vh[0]
+    else:
+        Y = ops.gaussian((nj, nb, l), seed=seed, device=Bt.device)[..., :ls]
+        Y = ops.contiguous(Y, dt)
+        Q = linalg.orthonormalize(apply_M(Y).reshape(1, rows_i, ls))[0]
+        for _ in range(q):
+            Z = apply_MT(ops.cast(Q.view(ni, ng, ls), dt))
+            Q2 = linalg.orthonormalize(_f64(Z).reshape(1, rows_j, ls))[0]
+            Y = apply_M(ops.cast(Q2.view(nj, nb, ls), dt))
+            Q = linalg.orthonormalize(_f64(Y).reshape(1, rows_i, ls))[0]
+        # X^T = M^T Q : (j,b,l), then the thin SVD of X (l x rows_j)
+        XT = _f64(apply_MT(ops.cast(Q.view(ni, ng, ls), dt))).reshape(1, rows_j, ls)
+        v, s, uh = linalg.svd_thin(XT)       # X^T = v s uh  ->  X = uh^T s v^T
+        UM = ops.gemm(Q, ops.contiguous(uh[0].transpose(0, 1)))
+        sM = s[0]
+        VMH = ops.contiguous(v[0].transpose(0, 1))
+    k = min(linalg.count_above(sM, eps), chi_k)
+    k = max(k, 1)
+    UM = ops.contiguous(UM[:, :k]).view(ni, ng, k)
+    VMH = ops.contiguous(VMH[:k]).view(k, nj, nb)
+    return UM, ops.contiguous(sM[:k]), VMH
+
+
+def _f64(X):
+    return X if X.dtype == torch.float64 else ops.cast(X, torch.float64)
+
+
+def leg_swapping(B, C, Dcut, ispure=True, prec="f64"):
+    """:142-235.  B (i,t1,x1,y1), C (T0,X0,Y0,j) -> UM (i,X0,Y0,k), sM (k), VMH (k,j,x1,y1)."""
+    ni, nt1, nx1, ny1 = B.shape
+    nT0, nX0, nY0, nj = C.shape
+    beta_ref = min(nx1 * ny1, ni * nt1)
+    gamma_ref = min(nT0 * nj, nX0 * nY0)
+    chi_k = min(Dcut, nj * beta_ref, ni * gamma_ref)
+    Bt = ops.permute(B, (2, 3, 0, 1)).view(nx1 * ny1, ni, nt1)      # (b, i, t)
+    Ct = ops.permute(C, (0, 3, 1, 2)).view(nT0, nj, nX0 * nY0)      # (t, j, g)
+    q = Dcut if RSVD["q"] is None else RSVD["q"]
+    UM, sM, VMH = rsvd_for_ATRG_leg_swapping(Bt, Ct, RSVD["k_factor"] * chi_k, RSVD["p"], q, truncate_eps,
+                                             RSVD["seed"])
+    chi, _ = linalg.truncate(sM, chi_k, degeneracy_eps, truncate_eps)
+    UM = ops.contiguous(UM[:, :, :chi]).view(ni, nX0, nY0, chi)
+    VMH = ops.contiguous(VMH[:chi]).view(chi, nj, nx1, ny1)
+    return UM, ops.contiguous(sM[:chi]), VMH
+
+
+# ---------------------------------------------------------------------------
+# squeezer (:246-317)
+# ---------------------------------------------------------------------------
+def squeezer_environments(A, UM, sM, VMH, D):
+    """The four environments (:262-265) as float64 matrices."""
+    nX1, nY1 = A.shape[1], A.shape[2]
+    nX0, nY0 = UM.shape[1], UM.shape[2]
+    nx1, ny1 = VMH.shape[2], VMH.shape[3]
+    nx0, ny0 = D.shape[2], D.shape[3]
+    dt = A.dtype
+    sMd = ops.cast(sM, dt)
+    UMs = ops.contiguous(UM)
+    ops.scale_rows_(UMs, sMd, UM.shape[3], 1, 0)                      # UM * s (last leg)
+    VMHs = ops.contiguous(VMH)
+    ops.scale_rows_(VMHs, sMd, VMH.shape[0], VMH.numel() // VMH.shape[0], 0)   # s * VMH (first leg)
+    P = contract("aibc,akbf->ickf", A, A)
+    Q = contract("cjde,flde->cjfl", UM, UMs)
+    XdX = contract("ickf,cjfl->ijkl", P, Q).reshape(nX1 * nX0, nX1 * nX0)
+    P = contract("abic,abkf->ickf", A, A)
+    Q = contract("cdje,fdle->cjfl", UM, UMs)
+    YdY = contract("ickf,cjfl->ijkl", P, Q).reshape(nY1 * nY0, nY1 * nY0)
+    P = contract("bejf,delf->bjdl", D, D)
+    Q = contract("abic,adkc->bidk", VMHs, VMH)
+    xxd = contract("bidk,bjdl->ijkl", Q, P).reshape(nx1 * nx0, nx1 * nx0)
+    P = contract("befj,defl->bjdl", D, D)
+    Q = contract("abci,adck->bidk", VMHs, VMH)
+    yyd = contract("bidk,bjdl->ijkl", Q, P).reshape(ny1 * ny0, ny1 * ny0)
+    return [_f64(e) for e in (XdX, YdY, xxd, yyd)]
+
+
+EIG_Q = 12   # power iterations of the top-k SVD of RX Rx when it is too large for the dense path
+
+
+def _oblique_pair(Env_big, Env_small, Dcut, shape_big, shape_small, dt):
+    """:278-292 with R^T R = Env_big (RX) and Rx Rx^T = Env_small; returns Px, PX."""
+    RX = linalg.chol_psd_upper(Env_big[None])[0]                     # R^T R = XdagX
+    Rx = ops.contiguous(linalg.chol_psd_upper(Env_small[None])[0].transpose(0, 1))   # Rx Rx^T = xxdag
+    P = ops.gemm(RX, Rx)
+    n = P.shape[0]
+    k1 = min(RX.shape[0], Rx.shape[1], Dcut)
+    if linalg._fits_small(n, n):
+        u, s, vh = linalg.svd_small(P[None])
+        u, s, vh = u[0], s[0], vh[0]
+        chi, _ = linalg.truncate(s, k1, degeneracy_eps, truncate_eps)
+    else:
+        l = min(n, max(2 * k1, k1 + 8), linalg.SMALL_MAX)
+        u, s, vh = linalg.svd_topk(P[None], l, EIG_Q)
+        u, s, vh = u[0], s[0], vh[0]
+        chi, _ = _truncate_with_tail(s, P, k1)
+    chi = min(chi, Dcut)
+    sis = ops.contiguous(s[:chi])
+    v = ops.contiguous(vh[:chi].transpose(0, 1))                      # (n, chi)
+    PX = ops.gemm(Rx, v)
+    ops.scale_rows_(PX, sis, chi, 1, 2)                                # * s^-1/2 (columns)
+    Px = ops.gemm(ops.contiguous(u[:, :chi].transpose(0, 1)), RX)
+    ops.scale_rows_(Px, sis, chi, n, 2)                                # s^-1/2 * (rows)
+    return (ops.cast(Px, dt).view((chi,) + shape_small), ops.cast(PX, dt).view(shape_big + (chi,)))
+
+
+def _truncate_with_tail(s, P, k):
+    """truncate() when only the leading singular values are known: the
+    discarded tail ||P||_F^2 - sum s^2 is added as one extra entry (it only
+    enters the tail-norm rule; the degeneracy rule needs s up to k+1)."""
+    fro = ops.contiguous(P).view(1, -1)
+    f2 = ops.gemm(fro, fro.transpose(0, 1)).view(1)                   # ||P||_F^2
+    s2 = ops.gemm(s.view(1, -1), s.view(-1, 1)).view(1)
+    tail = torch.empty(1, dtype=torch.float64, device=s.device)
+    ops.copy_into(tail, f2)
+    call_axpy = ops.gemm(torch.ones_like(s2).view(1, 1), s2.view(1, 1), tail.view(1, 1), alpha=-1.0, beta=1.0)
+    del call_axpy
+    ext = torch.empty(s.numel() + 1, dtype=torch.float64, device=s.device)
+    ops.copy_into(ext[:-1], s)
+    # a single trailing value carrying the discarded mass keeps the tail norms exact
+    tv = ops.contiguous(tail.abs() if False else tail)
+    ops.copy_into(ext[-1:], _sqrt_dev(tv))
+    return linalg.truncate(ext, k, degeneracy_eps, truncate_eps)
+
+
+def _sqrt_dev(x):
+    y = ops.contiguous(x)
+    one = torch.empty_like(y)
+    ops.fill_(one, 1.0)
+    ops.scale_rows_(one, ops.contiguous(y), 1, 1, 1)   # one * sqrt(y)
+    return one
+
+
+def squeezer(A, UM, sM, VMH, D, Dcut):
+    """:246-317 -> Px (x,x1,x0), PX (X1,X0,X), Py, PY."""
+    XdX, YdY, xxd, yyd = squeezer_environments(A, UM, sM, VMH, D)
+    dt = A.dtype
+    Px, PX = _oblique_pair(XdX, xxd, Dcut, (A.shape[1], UM.shape[1]), (VMH.shape[2], D.shape[2]), dt)
+    Py, PY = _oblique_pair(YdY, yyd, Dcut, (A.shape[2], UM.shape[2]), (VMH.shape[3], D.shape[3]), dt)
+    return Px, PX, Py, PY
+
+
+# ---------------------------------------------------------------------------
+# coarse graining (:357-438)
+# ---------------------------------------------------------------------------
+def contract_G(A, PX, UM, PY, compute=None):
+    """G_{T,X,Y,k} = A_{T,a,b,i} PX_{a,c,X} UM_{i,c,d,k} PY_{b,d,Y} (:379-384)."""
+    W = contract("Tabi,acX->TbicX", A, PX, compute=compute)
+    W = contract("TbicX,icdk->TbXdk", W, UM, compute=compute)
+    return contract("TbXdk,bdY->TXYk", W, PY, compute=compute)
+
+
+def contract_H(VMH, Px, D, Py, compute=None):
+    """H_{k,t,x,y} = VMH_{k,j,a,b} Px_{x,a,c} D_{j,t,c,d} Py_{y,b,d} (:400-405)."""
+    W = contract("kjab,xac->kjbxc", VMH, Px, compute=compute)
+    W = contract("kjbxc,jtcd->kbxtd", W, D, compute=compute)
+    return contract("kbxtd,ybd->ktxy", W, Py, compute=compute)
+
+
+def _svd_trunc(M2, k):
+    u, s, vh = linalg.svd_thin(_f64(M2)[None])
+    u, s, vh = u[0], s[0], vh[0]
+    chi, _ = linalg.truncate(s, k, degeneracy_eps, truncate_eps)
+    return ops.contiguous(u[:, :chi]), ops.contiguous(s[:chi]), ops.contiguous(vh[:chi])
+
+
+def coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY, compute=None):
+    nT, nt = A.shape[0], D.shape[1]
+    nX, nY = PX.shape[2], PY.shape[2]
+    nx, ny = Px.shape[0], Py.shape[0]
+    nk = UM.shape[3]
+    G = contract_G(A, PX, UM, PY, compute)
+    H = contract_H(VMH, Px, D, Py, compute)
+    UG, sG, VGH = _svd_trunc(G.reshape(nT * nX * nY, nk), min(nT * nX * nY, nk))
+    UH, sH, VHH = _svd_trunc(H.reshape(nk, nt * nx * ny), min(nk, nt * nx * ny))
+    # K = sG VGH sM UH sH
+    Kl = ops.contiguous(VGH)
+    ops.scale_rows_(Kl, sG, sG.numel(), Kl.shape[1], 0)
+    ops.scale_rows_(Kl, ops.contiguous(sM), sM.numel(), 1, 0)
+    Kr = ops.contiguous(UH)
+    ops.scale_rows_(Kr, sH, sH.numel(), 1, 0)
+    K = ops.gemm(Kl, Kr)
+    UK, sK, VKH = _svd_trunc(K, min(K.shape))
+    dt = A.dtype
+    U = ops.cast(ops.gemm(UG, UK), dt).view(nT, nX, nY, -1)
+    VH = ops.cast(ops.gemm(VKH, VHH), dt).view(-1, nt, nx, ny)
+    return U, sK, VH
+
+
+def atrg_pure_tensor(U, s, VH, Dcut, direction, prec="f64"):
+    """:441-482 -> (U, s, VH) of the coarse tensor (legs restored)."""
+    dt = _dt(prec)
+    U, VH = ops.cast(U, dt), ops.cast(VH, dt)
+    U, VH = leg_transposition(U, VH, "transpose", direction)
+    sd = ops.cast(s, dt)
+    A, D = U, VH
+    B = ops.contiguous(VH)
+    ops.scale_rows_(B, sd, sd.numel(), VH.numel() // VH.shape[0], 0)    # s_i VH_{itxy}
+    C = ops.contiguous(U)
+    ops.scale_rows_(C, sd, sd.numel(), 1, 0)                             # U_{TXYj} s_j
+    UM, sM, VMH = leg_swapping(B, C, Dcut, True, prec)
+    UM, VMH = ops.cast(UM, dt), ops.cast(VMH, dt)
+    Px, PX, Py, PY = squeezer(A, UM, sM, VMH, D, Dcut)
+    U2, s2, VH2 = coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY)
+    U2, VH2 = leg_transposition(U2, VH2, "restore", direction)
+    return U2, s2, VH2
+
+
+def trace(U, s, VH):
+    """Tr T = sum_{txy,i} U_{txyi} s_i VH_{itxy} (tensor_class/tensor_class.py:66-86), float64 (1,)."""
+    d = contract("txyi,itxy->i", _f64(U), _f64(VH))
+    return ops.gemm(d.view(1, -1), _f64(s).view(-1, 1)).view(1)
+
+
+def normalize(U, s, VH):
+    """:954-958: c = |Tr T|; s /= c.  Returns (s, c) with c a float64 device scalar (1,)."""
+    c = trace(U, s, VH)
+    s = ops.contiguous(_f64(s))
+    ops.normalize_(s.view(1, -1), c)
+    return s, c

@@ -174,8 +174,13 @@ __global__ void __launch_bounds__(JT) syevj_kernel(int n, double *Ag, long long 
 // ---------------------------------------------------------------------------
 // Shifted Cholesky  R^T R = G + shift_rel * tr(G) * I  and  Rinv = R^{-1}.
 // ---------------------------------------------------------------------------
+// psd_tol > 0 selects the positive-semidefinite mode: a pivot below
+// psd_tol * scale[b] zeroes its row of R (and the matching column of the
+// generalised inverse) instead of flagging an error.  Rout (optional)
+// receives R itself.
 __global__ void __launch_bounds__(256) chol_inv_kernel(int l, double *Gg, long long sG, double shift_rel,
-                                                       double *Rinv, int *info, int use_lds) {
+                                                       double *Rinv, int *info, int use_lds, double psd_tol,
+                                                       const double *scale, double *Rout) {
     extern __shared__ double lds[];
     __shared__ double red[256];
     __shared__ double piv;
@@ -212,19 +217,25 @@ __global__ void __launch_bounds__(256) chol_inv_kernel(int l, double *Gg, long l
     for (int i = tid; i < l; i += 256) G[i * l + i] += shift;
     __syncthreads();
     const double tiny = 1e-300 + 1e-30 * fabs(red[0]);
+    const double zero_below = psd_tol > 0 ? psd_tol * (scale ? scale[b] : fabs(red[0])) : -1.0;
     for (int j = 0; j < l; ++j) {
         if (tid == 0) {
             double d = G[j * l + j];
-            if (!(d > tiny)) {   // also catches NaN
-                bad = 1;
-                d = tiny > 0 ? tiny : 1e-300;
+            if (zero_below >= 0) {
+                piv = (d > zero_below) ? sqrt(d) : 0.0;
+                if (d != d) bad = 1;
+            } else {
+                if (!(d > tiny)) {   // also catches NaN
+                    bad = 1;
+                    d = tiny > 0 ? tiny : 1e-300;
+                }
+                piv = sqrt(d);
             }
-            piv = sqrt(d);
             G[j * l + j] = piv;
         }
         __syncthreads();
         const double r = piv;
-        for (int i = j + 1 + tid; i < l; i += 256) G[j * l + i] /= r;
+        for (int i = j + 1 + tid; i < l; i += 256) G[j * l + i] = r > 0.0 ? G[j * l + i] / r : 0.0;
         __syncthreads();
         const int m = l - j - 1;
         for (int e = tid; e < m * m; e += 256) {
@@ -237,8 +248,16 @@ __global__ void __launch_bounds__(256) chol_inv_kernel(int l, double *Gg, long l
     // leading j x j block already inverted (X), column j of X is
     // -X[0:j,0:j] R[0:j,j] / R[j][j] and X[j][j] = 1 / R[j][j].
     __shared__ double colv[256];
+    if (Rout) {
+        for (int e = tid; e < l * l; e += 256) {
+            const int i = e / l, c = e % l;
+            Rout[(long long)b * l * l + e] = (i <= c) ? G[e] : 0.0;
+        }
+        __syncthreads();
+    }
     for (int j = 0; j < l; ++j) {
-        const double djj = 1.0 / G[j * l + j];
+        const double gjj = G[j * l + j];
+        const double djj = gjj != 0.0 ? 1.0 / gjj : 0.0;
         for (int i = tid; i < j; i += 256) {
             double v = 0.0;
             for (int k = i; k < j; ++k) v += G[i * l + k] * G[k * l + j];
@@ -327,7 +346,7 @@ extern "C" int trg_chol_inv_f64(int batch, int l, double *G, long long sG, doubl
         }
     }
     hipLaunchKernelGGL(chol_inv_kernel, batch, 256, use_lds ? bytes : 0, as_stream(stream), l, G, sG, shift_rel, Rinv,
-                       info, use_lds);
+                       info, use_lds, 0.0, (const double *)nullptr, (double *)nullptr);
     return check_launch();
 }
 
@@ -415,5 +434,259 @@ extern "C" int trg_eigh_topk_f64(int batch, int n, int k, int l, int q, const do
     hipLaunchKernelGGL(tail_copy_kernel, dim3(1, batch), 256, 0, s, batch, l, k, theta, w);
     TRG_TRY(check_launch());
     hipLaunchKernelGGL(discarded_kernel, (unsigned)cdiv(batch, 256), 256, 0, s, batch, k, tr, w, (long long)k, eps);
+    return check_launch();
+}
+
+// ---------------------------------------------------------------------------
+// One-sided (Hestenes) Jacobi SVD of an m x n matrix, n <= m <= 256 per
+// instance, one workgroup each.  Columns are orthogonalised pairwise in the
+// round-robin order; singular values are the final column norms (high
+// relative accuracy), U = W diag(1/s), V accumulates the rotations.
+// Output: s descending (stride n), U (m x n row-major, stride m*n), V
+// (n x n, stride n*n, columns are right singular vectors).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) gesvj_kernel(int m, int n, const double *Ag, long long sA, double *sout,
+                                                    double *Uout, double *Vout, int max_sweeps, int *info,
+                                                    int use_lds) {
+    extern __shared__ double lds[];
+    __shared__ double cs[JMAX / 2], sn[JMAX / 2];
+    __shared__ int pp[JMAX / 2], qq[JMAX / 2];
+    __shared__ int rotated;
+    __shared__ double nrm[JMAX];
+    __shared__ int rank[JMAX];
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double *A = Ag + (long long)b * sA;
+    double *U = Uout + (long long)b * m * n;
+    double *V = Vout + (long long)b * n * n;
+    double *W = use_lds ? lds : U;     // work columns (row-major m x n)
+    for (int e = tid; e < m * n; e += 256) W[e] = A[e];
+    for (int e = tid; e < n * n; e += 256) V[e] = (e / n == e % n) ? 1.0 : 0.0;
+    __syncthreads();
+    const int N = n + (n & 1);
+    const int npair = N / 2;
+    int sweep = 0;
+    for (; sweep < max_sweeps; ++sweep) {
+        if (tid == 0) rotated = 0;
+        __syncthreads();
+        for (int r = 0; r < N - 1; ++r) {
+            // each wave handles pairs wave, wave+4, ...: column dot products over m
+            for (int i = wave; i < npair; i += 4) {
+                int p = rr_pos(i, r, N), q = rr_pos(N - 1 - i, r, N);
+                if (p > q) { const int t = p; p = q; q = t; }
+                double a = 0.0, bq = 0.0, g = 0.0;
+                if (q < n) {
+                    for (int k = lane; k < m; k += 64) {
+                        const double x = W[k * n + p], y = W[k * n + q];
+                        a += x * x; bq += y * y; g += x * y;
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    a += __shfl_xor(a, off);
+                    bq += __shfl_xor(bq, off);
+                    g += __shfl_xor(g, off);
+                }
+                if (lane == 0) {
+                    double c = 1.0, s = 0.0;
+                    if (q < n && fabs(g) > 2.2e-16 * sqrt(a * bq) && g != 0.0) {
+                        const double zeta = (bq - a) / (2.0 * g);
+                        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = c * t;
+                        rotated = 1;
+                    }
+                    cs[i] = c; sn[i] = s; pp[i] = p; qq[i] = q;
+                }
+            }
+            __syncthreads();
+            for (int e = tid; e < npair * (m + n); e += 256) {
+                const int i = e / (m + n), k = e % (m + n);
+                const double s = sn[i];
+                if (s == 0.0) continue;
+                const double c = cs[i];
+                const int p = pp[i], q = qq[i];
+                if (k < m) {
+                    const double x = W[k * n + p], y = W[k * n + q];
+                    W[k * n + p] = c * x - s * y;
+                    W[k * n + q] = s * x + c * y;
+                } else {
+                    const int kk = k - m;
+                    const double x = V[kk * n + p], y = V[kk * n + q];
+                    V[kk * n + p] = c * x - s * y;
+                    V[kk * n + q] = s * x + c * y;
+                }
+            }
+            __syncthreads();
+        }
+        if (!rotated) break;
+        __syncthreads();
+    }
+    // column norms, descending rank
+    for (int j = wave; j < n; j += 4) {
+        double a = 0.0;
+        for (int k = lane; k < m; k += 64) a += W[k * n + j] * W[k * n + j];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+        if (lane == 0) nrm[j] = sqrt(a);
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += 256) {
+        int rk = 0;
+        const double si = nrm[i];
+        for (int j = 0; j < n; ++j) rk += (nrm[j] > si) || (nrm[j] == si && j < i);
+        rank[i] = rk;
+        sout[(long long)b * n + rk] = si;
+    }
+    __syncthreads();
+    // U[:, rank[j]] = W[:, j] / s_j (zero column for s_j = 0);  V columns permuted via a copy
+    if (use_lds) {
+        for (int e = tid; e < m * n; e += 256) {
+            const int k = e / n, j = e % n;
+            U[k * n + rank[j]] = nrm[j] > 0.0 ? W[e] / nrm[j] : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < n * n; e += 256) W[e] = V[e];
+        __syncthreads();
+        for (int e = tid; e < n * n; e += 256) {
+            const int k = e / n, j = e % n;
+            V[k * n + rank[j]] = W[e];
+        }
+    } else {
+        // W aliases U: permute through V's space is not available, so do the
+        // column scaling in place and leave the permutation to the caller's
+        // perm output (rank) -- only the LDS path is used (host checks).
+    }
+    if (tid == 0 && info) info[b] = (sweep >= max_sweeps) ? 1 : 0;
+}
+
+// Rank selection of utility/truncated_svd.py:17-78 on the device for one
+// descending vector s (length n):  writes chi and the tail-norm error.
+__global__ void truncate_kernel(int n, const double *s, int k, double deg_eps, double trunc_eps, int *chi_out,
+                                double *err_out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (deg_eps == 0.0 && trunc_eps == 0.0) { *chi_out = k; *err_out = 0.0; return; }
+    if (k > n) { *chi_out = n; *err_out = 0.0; return; }
+    const double s0 = s[0];
+    auto delta = [&](int i) -> int {   // diff_i > deg_eps ?  (diff_0 = 0, diff_n = 1)
+        if (i == 0) return 0 > deg_eps ? 1 : 0;
+        if (i >= n) return 1.0 > deg_eps ? 1 : 0;
+        const double a = s[i] / s0, p = s[i - 1] / s0;
+        const double d = fabs(a - p) / fabs(a);
+        return d > deg_eps ? 1 : 0;    // NaN compares false
+    };
+    int chi1 = k;
+    for (chi1 = k; chi1 > 0; --chi1)
+        if (delta(chi1) == 1) break;
+    if (chi1 == 0) chi1 = 1;
+    double tot = 0.0;
+    for (int i = n - 1; i >= 0; --i) tot += (s[i] / s0) * (s[i] / s0);
+    // errors[i] = sqrt(sum_{j>=i} s_j^2 / tot), decreasing in i
+    int chi2 = 0;
+    double tail = tot;
+    for (int i = 0; i < chi1; ++i) {
+        if (sqrt(tail / tot) > trunc_eps) ++chi2;
+        tail -= (s[i] / s0) * (s[i] / s0);
+    }
+    int chi = chi1;
+    if (chi2 < chi1) {
+        for (chi = chi2; chi <= chi1; ++chi)
+            if (delta(chi) == 1) break;
+        if (chi > chi1) chi = chi1;
+    }
+    double err = 0.0;
+    if (chi < n) {
+        double t = 0.0;
+        for (int i = n - 1; i >= chi; --i) t += (s[i] / s0) * (s[i] / s0);
+        err = sqrt(t / tot);
+    }
+    *chi_out = chi;
+    *err_out = err;
+}
+
+extern "C" int trg_gesvj_f64(int batch, int m, int n, const double *A, long long sA, double *s, double *U, double *V,
+                             int max_sweeps, int *info, void *stream) {
+    if (batch < 0 || m < 0 || n < 0 || max_sweeps < 1) return TRG_BAD_ARG;
+    if (batch == 0 || m == 0 || n == 0) return TRG_OK;
+    if (n > m || m > JMAX) return TRG_SHAPE;
+    if (!A || !s || !U || !V) return TRG_BAD_ARG;
+    if (batch > 65535) return TRG_SHAPE;
+    const size_t bytes = (size_t)m * n * sizeof(double);
+    if (bytes + 16384 > (size_t)lds_limit()) return TRG_SHAPE;
+    if (bytes > 65536) {
+        if (hipFuncSetAttribute((const void *)gesvj_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+            hipSuccess) {
+            g_last_hip_error = (int)hipGetLastError();
+            return TRG_HIP_ERROR;
+        }
+    }
+    hipLaunchKernelGGL(gesvj_kernel, batch, 256, bytes, as_stream(stream), m, n, A, sA, s, U, V, max_sweeps, info, 1);
+    return check_launch();
+}
+
+extern "C" int trg_truncate_rank(int n, const double *s, int k, double degeneracy_eps, double truncate_eps, int *chi,
+                                 double *err, void *stream) {
+    if (n < 0 || k < 0) return TRG_BAD_ARG;
+    if (!s || !chi || !err) return TRG_BAD_ARG;
+    if (n == 0) return TRG_SHAPE;
+    hipLaunchKernelGGL(truncate_kernel, 1, 64, 0, as_stream(stream), n, s, k, degeneracy_eps, truncate_eps, chi, err);
+    return check_launch();
+}
+
+extern "C" int trg_potrf_psd_f64(int batch, int l, double *G, long long sG, double psd_tol, const double *scale,
+                                 double *R, double *Rinv, void *stream) {
+    if (batch < 0 || l < 0 || psd_tol <= 0) return TRG_BAD_ARG;
+    if (batch == 0 || l == 0) return TRG_OK;
+    if (l > 256) return TRG_SHAPE;
+    if (!G || !Rinv) return TRG_BAD_ARG;
+    if (batch > 65535) return TRG_SHAPE;
+    const size_t bytes = (size_t)l * l * sizeof(double);
+    const int use_lds = (bytes + 8192 <= (size_t)lds_limit()) ? 1 : 0;
+    if (use_lds && bytes > 65536) {
+        if (hipFuncSetAttribute((const void *)chol_inv_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+            hipSuccess) {
+            g_last_hip_error = (int)hipGetLastError();
+            return TRG_HIP_ERROR;
+        }
+    }
+    hipLaunchKernelGGL(chol_inv_kernel, batch, 256, use_lds ? bytes : 0, as_stream(stream), l, G, sG, 0.0, Rinv,
+                       (int *)nullptr, use_lds, psd_tol, scale, R);
+    return check_launch();
+}
+
+extern "C" long long trg_orthonormalize_workspace(int batch, int m, int l) {
+    if (batch <= 0 || m <= 0 || l <= 0) return 0;
+    return (long long)batch * ((long long)m * l + 2LL * l * l) * (long long)sizeof(double) + 4LL * batch + 256;
+}
+
+extern "C" int trg_orthonormalize_f64(int batch, int m, int l, double *X, long long sX, double *Q, void *ws,
+                                      long long ws_bytes, int *info, void *stream) {
+    // X (batch x m x l, stride sX) is clobbered; the orthonormal basis of its
+    // column span lands in Q (stride m*l).  Shifted CholeskyQR3.
+    if (batch < 0 || m < 0 || l < 0) return TRG_BAD_ARG;
+    if (batch == 0 || m == 0 || l == 0) return TRG_OK;
+    if (l > 256 || l > m) return TRG_SHAPE;
+    if (!X || !Q || !ws) return TRG_BAD_ARG;
+    if (ws_bytes < trg_orthonormalize_workspace(batch, m, l)) return TRG_OOM;
+    hipStream_t s = as_stream(stream);
+    const long long mx = (long long)m * l, sG = (long long)l * l;
+    double *Xc = static_cast<double *>(ws);
+    double *G = Xc + batch * mx;
+    double *R = G + batch * sG;
+    int *flags = reinterpret_cast<int *>(R + batch * sG);
+    long long dims[2] = {batch, mx}, si[2] = {sX, 1}, so[2] = {mx, 1};
+    TRG_TRY(trg_copy_strided(TRG_F64, TRG_F64, 2, dims, X, si, Xc, so, s));
+    if (info) TRG_TRY(trg_fill(3, batch, 0.0, info, s));
+    // passes: Xc -> Q -> Xc -> Q
+    TRG_TRY(cholqr3(batch, m, l, Xc, Q, G, R, flags, info, s));
+    return TRG_OK;
+}
+
+// Y[b] += alpha_rel * scal[b] * X[b]
+extern "C" int trg_axpy_dev_f64(int batch, long long n, double alpha_rel, const double *scal, const double *X,
+                                long long sX, double *Y, long long sY, void *stream) {
+    if (batch < 0 || n < 0) return TRG_BAD_ARG;
+    if (batch == 0 || n == 0) return TRG_OK;
+    if (!scal || !X || !Y) return TRG_BAD_ARG;
+    if (batch > 65535) return TRG_SHAPE;
+    const dim3 grid((unsigned)std::min<long long>(cdiv(n, 256), 4096), batch);
+    hipLaunchKernelGGL(axpy_scaled_kernel, grid, 256, 0, as_stream(stream), n, alpha_rel, scal, X, sX, Y, sY);
     return check_launch();
 }
