@@ -31,6 +31,17 @@ truncate_eps = 1e-10
 _RSVD = {"k_factor": 3, "p": 0, "q": None, "seed": 12345}
 
 
+# Basis of the leg swap: "svd" follows the reference (SVDs of B and C,
+# trg/ATRG_3d_new.py:175-188); "identity" keeps the (x1,y1) / (X0,Y0) legs
+# as they are.  Both give the same M = B~ C~ up to a rotation of its row and
+# column legs; "identity" draws the rSVD sketch in the same basis as the GPU.
+LEG_SWAP_BASIS = {"mode": "svd"}
+
+
+def set_leg_swap_basis(mode):
+    LEG_SWAP_BASIS["mode"] = mode
+
+
 def set_rsvd_params(k_factor=3, p=0, q=None, seed=12345):
     """Select the leg-swap rSVD parameters (reference: 3, 0, Dcut, 12345)."""
     _RSVD.update(k_factor=k_factor, p=p, q=q, seed=seed)
@@ -108,6 +119,8 @@ def leg_swapping(B, C, Dcut, ispure=True):
     UM_{i,X0,Y0,k}, sM_k, VMH_{k,j,x1,y1}."""
     ni, nt1, nx1, ny1 = B.shape
     nT0, nX0, nY0, nj = C.shape
+    if LEG_SWAP_BASIS["mode"] == "identity":
+        return _leg_swapping_identity(B, C, Dcut)
     Bm = np.transpose(B, (2, 3, 0, 1)).reshape(nx1 * ny1, ni * nt1)
     UB, sB, VBH = np.linalg.svd(Bm, full_matrices=False)
     beta = len(sB)
@@ -128,6 +141,24 @@ def leg_swapping(B, C, Dcut, ispure=True):
     UM = np.transpose(UM, (0, 2, 3, 1))
     VMH = np.tensordot(VMH, UB, axes=([2], [2]))            # (k,j,x,y)
     return UM, sM, VMH
+
+
+def _leg_swapping_identity(B, C, Dcut):
+    """leg_swapping with the (x1,y1) and (X0,Y0) legs kept as the beta and
+    gamma bases; chi_k is still bounded by the reference's reduced sizes."""
+    ni, nt1, nx1, ny1 = B.shape
+    nT0, nX0, nY0, nj = C.shape
+    beta_ref = min(nx1 * ny1, ni * nt1)
+    gamma_ref = min(nT0 * nj, nX0 * nY0)
+    chi_k = min(Dcut, nj * beta_ref, ni * gamma_ref)
+    Bt = np.ascontiguousarray(np.transpose(B, (2, 3, 0, 1))).reshape(nx1 * ny1, ni, nt1)
+    Ct = np.ascontiguousarray(np.transpose(C, (0, 3, 1, 2))).reshape(nT0, nj, nX0 * nY0)
+    q = Dcut if _RSVD["q"] is None else _RSVD["q"]
+    UM, sM, VMH = rsvd_for_ATRG_leg_swapping(Bt, Ct, chi_k=_RSVD["k_factor"] * chi_k,
+                                             p=_RSVD["p"], q=q, eps=truncate_eps, seed=_RSVD["seed"])
+    chi, _ = truncate(sM, k=chi_k, degeneracy_eps=degeneracy_eps, truncate_eps=truncate_eps)
+    UM, sM, VMH = UM[:, :, :chi], sM[:chi], VMH[:chi]
+    return UM.reshape(ni, nX0, nY0, chi), sM, VMH.reshape(chi, nj, nx1, ny1)
 
 
 def squeezer_environments(A, UM, sM, VMH, D):

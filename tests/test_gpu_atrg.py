@@ -1,0 +1,191 @@
+"""GPU tests: decompositions, utility drop-ins and ATRG 3D parity."""
+import warnings
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ATRG_3d_new as oat
+from oracle import exact, models
+from oracle import truncated_svd as otsvd
+
+pytestmark = pytest.mark.gpu
+warnings.filterwarnings("ignore")
+
+
+def _dev(x, gpu):
+    return torch.as_tensor(np.ascontiguousarray(x)).to(gpu)
+
+
+@pytest.mark.parametrize("m,n", [(5, 3), (64, 64), (128, 128), (200, 40), (256, 64)])
+def test_gesvj_matches_lapack(gpu, m, n):
+    from trg_backup_amd import linalg
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, n)) * np.exp(-np.arange(n) / 8.0)[None, :]
+    u, s, vh = linalg.svd_small(_dev(A, gpu)[None])
+    u, s, vh = u[0].cpu().numpy(), s[0].cpu().numpy(), vh[0].cpu().numpy()
+    ref = np.linalg.svd(A, compute_uv=False)
+    assert np.allclose(s, ref, rtol=1e-12, atol=1e-14 * ref[0])
+    assert np.allclose((u * s) @ vh, A, atol=1e-12 * ref[0])
+    assert np.allclose(vh @ vh.T, np.eye(n), atol=1e-12)
+
+
+def test_truncate_device_matches_oracle(gpu):
+    from trg_backup_amd import linalg
+    rng = np.random.default_rng(0)
+    cases = [np.array([1.0, 0.5, 0.5, 0.1, 1e-12]), np.sort(rng.random(40))[::-1],
+             np.concatenate([np.ones(3), 0.3 * np.ones(4), np.geomspace(0.1, 1e-14, 20)])]
+    for s in cases:
+        for k in (1, 2, 3, 5, 7, len(s), len(s) + 3):
+            for de, te in ((0.0, 1e-10), (1e-3, 1e-10), (0.0, 0.0), (5e-2, 0.0), (1e-3, 1e-3)):
+                ref = otsvd.truncate(s, k, de, te)
+                got = linalg.truncate(_dev(s, gpu), k, de, te)
+                assert got[0] == ref[0], (s[:6], k, de, te, got, ref)
+                assert abs(got[1] - ref[1]) <= 1e-12 + 1e-12 * abs(ref[1])
+
+
+@pytest.mark.parametrize("n,r", [(100, 100), (300, 40), (600, 250)])
+def test_chol_psd_upper(gpu, n, r):
+    from trg_backup_amd import linalg
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, r)) * np.exp(-np.arange(r) / 10.0)[None, :]
+    A = X @ X.T
+    R = linalg.chol_psd_upper(_dev(A, gpu)[None])[0].cpu().numpy()
+    assert np.allclose(np.tril(R, -1), 0)
+    assert np.abs(R.T @ R - A).max() < 1e-12 * np.abs(A).max()
+
+
+def test_svd_thin_and_topk(gpu):
+    from trg_backup_amd import linalg
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((5000, 40)) @ np.diag(np.geomspace(1, 1e-6, 40)) @ rng.standard_normal((40, 40))
+    u, s, vh = linalg.svd_thin(_dev(A, gpu)[None])
+    s = s[0].cpu().numpy()
+    assert np.allclose(s, np.linalg.svd(A, compute_uv=False), rtol=1e-10)
+    rec = (u[0].cpu().numpy() * s) @ vh[0].cpu().numpy()
+    assert np.abs(rec - A).max() < 1e-12 * np.abs(A).max()
+    lam = np.geomspace(1, 1e-12, 300)
+    Q1, _ = np.linalg.qr(rng.standard_normal((400, 300)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((350, 300)))
+    B = (Q1 * lam) @ Q2.T
+    u, s, vh = linalg.svd_topk(_dev(B, gpu)[None], 64, 6)
+    assert np.allclose(s[0].cpu().numpy()[:32], lam[:32], rtol=1e-9)
+
+
+def test_utility_svd_eigh_dropins(gpu):
+    from trg_backup_amd.utility import truncated_svd as tsvd
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((4, 5, 6))
+    u, s, vh = tsvd.svd(A, shape=[[0, 2], [1]], k=5, truncate_eps=1e-14, degeneracy_eps=0)
+    uo, so, vo = otsvd.svd(A, shape=[[0, 2], [1]], k=5, truncate_eps=1e-14, degeneracy_eps=0)
+    assert u.shape == uo.shape and np.allclose(s, so, rtol=1e-12)
+    assert np.allclose(np.einsum("aci,i,ib->abc", u, s, vh), A, atol=1e-12)
+    H = rng.standard_normal((6, 6))
+    H = H @ H.T
+    e, v = tsvd.eigh(H.reshape(2, 3, 2, 3), shape=[[0, 1], [2, 3]], k=6, degeneracy_eps=0, truncate_eps=1e-15)
+    eo, vo = otsvd.eigh(H.reshape(2, 3, 2, 3), shape=[[0, 1], [2, 3]], k=6, degeneracy_eps=0, truncate_eps=1e-15)
+    assert np.allclose(e, eo, rtol=1e-12)
+
+
+def test_rsvd_dropin_exact_when_sketch_covers_rank(gpu):
+    from trg_backup_amd.utility import randomized_svd as rs
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((300, 8)) @ rng.standard_normal((8, 200))
+    u, s, vh = rs.rsvd(A, k=8, n_oversamples=8, n_power_iter=1, seed=3)
+    assert np.allclose(s, np.linalg.svd(A, compute_uv=False)[:8], rtol=1e-10)
+    assert np.abs((u * s) @ vh - A).max() < 1e-10 * np.abs(A).max()
+
+
+def test_atrg_ising_untruncated_equals_brute_force(gpu):
+    from trg_backup_amd.tensor_class.tensor_class import ATRG_Tensor
+    from trg_backup_amd.trg import ATRG_3d_new as at
+    beta = 0.22
+    T = models.to_atrg(models.ising3d_tensor(beta), 64)
+    A = ATRG_Tensor(T.U, T.s, T.VH, 64, 3, False, {})
+    at.set_rsvd_params(k_factor=2, p=0, q=2)
+    try:
+        v, _ = at.ln_Z_over_V(A, 64, {"X": 1, "Y": 1, "T": 1})
+    finally:
+        at.set_rsvd_params()
+    assert abs(v - exact.brute_lnZ_ising3d(beta)) < 1e-12
+
+
+def test_atrg_beta0_is_ln3(gpu):
+    from trg_backup_amd.tensor_class.tensor_class import ATRG_Tensor
+    from trg_backup_amd.trg import ATRG_3d_new as at
+    T = models.to_atrg(models.potts3d_tensor(0.0), 12)
+    v, _ = at.ln_Z_over_V(ATRG_Tensor(T.U, T.s, T.VH, 12, 3, False, {}), 12, {"X": 2, "Y": 2, "T": 2})
+    assert abs(v - np.log(3)) < 1e-12
+
+
+@pytest.mark.parametrize("model,beta,chi", [("ising", 0.22, 8), ("potts", 0.55, 10)])
+def test_atrg_matches_oracle_identity_basis(gpu, model, beta, chi):
+    """Same algorithm and the same Philox sketch on both sides (leg swap in
+    the identity basis, sketch of 2 chi columns all handed to truncate, q = 4
+    power iterations).  DGENERACY_EPS = 1e-8 on both sides: with 0 the rank
+    rule may cut inside a multiplet split only by rounding (Z2 / S3
+    symmetry), and which partner survives is then arbitrary in any two
+    implementations (LAPACK vs Jacobi)."""
+    from trg_backup_amd import atrg
+    from trg_backup_amd.tensor_class.tensor_class import ATRG_Tensor
+    from trg_backup_amd.trg import ATRG_3d_new as at
+    T6 = models.ising3d_tensor(beta) if model == "ising" else models.potts3d_tensor(beta)
+    steps = {"X": 2, "Y": 2, "T": 2}
+    oat.set_leg_swap_basis("identity")
+    # q = 20: the range finder converges to rounding, so the result no longer
+    # depends on the sketch (the two sides' tensors differ by a sign gauge of
+    # the internal bond, which a fixed Gaussian sketch sees)
+    oat.set_rsvd_params(k_factor=2, p=0, q=20, seed=1)
+    at.set_rsvd_params(k_factor=2, p=0, q=20, seed=1)
+    old = (oat.degeneracy_eps, atrg.degeneracy_eps)
+    oat.degeneracy_eps = atrg.degeneracy_eps = 1e-8
+    try:
+        ref, To = oat.ln_Z_over_V(models.to_atrg(T6, chi), chi, dict(steps))
+        Tg = models.to_atrg(T6, chi)
+        v, Tn = at.ln_Z_over_V(ATRG_Tensor(Tg.U, Tg.s, Tg.VH, chi, 3, False, {}), chi, dict(steps))
+        ref_svd_basis = None
+        oat.set_leg_swap_basis("svd")
+        ref_svd_basis, _ = oat.ln_Z_over_V(models.to_atrg(T6, chi), chi, dict(steps))
+    finally:
+        oat.set_leg_swap_basis("svd")
+        oat.set_rsvd_params()
+        at.set_rsvd_params()
+        oat.degeneracy_eps, atrg.degeneracy_eps = old
+    print(f"\n[atrg {model} chi={chi}] gpu={v!r} oracle(identity)={ref!r} diff={v - ref:.3e} "
+          f"oracle(svd basis)={ref_svd_basis!r} basis diff={ref_svd_basis - ref:.3e}", flush=True)
+    assert Tn.U.shape == To.U.shape
+    assert abs(v - ref) < 1e-11 * abs(ref)
+    # the reference's SVD basis draws the sketch in rotated legs: with a converged
+    # range finder it agrees to rounding as well
+    assert abs(ref_svd_basis - ref) < 1e-10 * abs(ref)
+
+
+@pytest.mark.parametrize("q", [2, 4])
+def test_atrg_few_power_iterations_within_rsvd_accuracy(gpu, q):
+    """The workload's setting (sketch 2 chi, q = 2): device and oracle agree
+    to the randomized SVD's own accuracy (they see the sketch in different
+    sign gauges of the internal bond)."""
+    from trg_backup_amd import atrg
+    from trg_backup_amd.tensor_class.tensor_class import ATRG_Tensor
+    from trg_backup_amd.trg import ATRG_3d_new as at
+    chi = 10
+    T6 = models.potts3d_tensor(0.55)
+    steps = {"X": 2, "Y": 2, "T": 2}
+    oat.set_leg_swap_basis("identity")
+    oat.set_rsvd_params(k_factor=2, p=0, q=q, seed=1)
+    at.set_rsvd_params(k_factor=2, p=0, q=q, seed=1)
+    old = (oat.degeneracy_eps, atrg.degeneracy_eps)
+    oat.degeneracy_eps = atrg.degeneracy_eps = 1e-8
+    try:
+        ref, _ = oat.ln_Z_over_V(models.to_atrg(T6, chi), chi, dict(steps))
+        oat.set_rsvd_params(k_factor=2, p=0, q=20, seed=1)
+        conv, _ = oat.ln_Z_over_V(models.to_atrg(T6, chi), chi, dict(steps))
+        Tg = models.to_atrg(T6, chi)
+        v, _ = at.ln_Z_over_V(ATRG_Tensor(Tg.U, Tg.s, Tg.VH, chi, 3, False, {}), chi, dict(steps))
+    finally:
+        oat.set_leg_swap_basis("svd")
+        oat.set_rsvd_params()
+        at.set_rsvd_params()
+        oat.degeneracy_eps, atrg.degeneracy_eps = old
+    print(f"\n[atrg potts q={q}] gpu-oracle={v - ref:.3e} oracle(q)-oracle(converged)={ref - conv:.3e}", flush=True)
+    assert abs(v - ref) <= 10 * abs(ref - conv) + 1e-12

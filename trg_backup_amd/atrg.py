@@ -75,9 +75,14 @@ def rsvd_for_ATRG_leg_swapping(Bt, Ct, chi_k, p, q, eps, seed):
         W = contract("bit,igl->btgl", Bt, Qigl)
         return contract("tjg,btgl->jbl", Ct, W)
 
-    if ls == rows_i or ls == rows_j or ls > linalg.SMALL_MAX:
-        if max(rows_i, rows_j) > 256 or min(rows_i, rows_j) > linalg.SMALL_MAX:
-            raise TrgError(f"leg-swap rSVD block {ls} beyond the device limits ({rows_i}x{rows_j})")
+    # The small SVDs run in one workgroup's LDS, which caps the sketch block
+    # at linalg.SMALL_MAX columns (a documented device limit: above it the
+    # block is truncated to SMALL_MAX).
+    if ls >= min(rows_i, rows_j) and min(rows_i, rows_j) > linalg.SMALL_MAX:
+        ls = linalg.SMALL_MAX
+    elif ls > linalg.SMALL_MAX:
+        ls = linalg.SMALL_MAX
+    if ls >= min(rows_i, rows_j):
         # the sketch spans the whole space: dense SVD of M (small early steps)
         M = contract("bit,tjg->igjb", Bt, Ct).reshape(rows_i, rows_j)
         u, s, vh = linalg.svd_thin(M[None])
@@ -186,30 +191,27 @@ def _oblique_pair(Env_big, Env_small, Dcut, shape_big, shape_small, dt):
 
 
 def _truncate_with_tail(s, P, k):
-    """truncate() when only the leading singular values are known: the
-    discarded tail ||P||_F^2 - sum s^2 is added as one extra entry (it only
-    enters the tail-norm rule; the degeneracy rule needs s up to k+1)."""
-    fro = ops.contiguous(P).view(1, -1)
-    f2 = ops.gemm(fro, fro.transpose(0, 1)).view(1)                   # ||P||_F^2
-    s2 = ops.gemm(s.view(1, -1), s.view(-1, 1)).view(1)
-    tail = torch.empty(1, dtype=torch.float64, device=s.device)
-    ops.copy_into(tail, f2)
-    call_axpy = ops.gemm(torch.ones_like(s2).view(1, 1), s2.view(1, 1), tail.view(1, 1), alpha=-1.0, beta=1.0)
-    del call_axpy
-    ext = torch.empty(s.numel() + 1, dtype=torch.float64, device=s.device)
+    """truncate() when only the leading singular values of P are known: the
+    discarded mass ||P||_F^2 - sum s_i^2 enters as one extra trailing value,
+    so the tail-norm rule sees the exact total (the degeneracy rule only
+    looks at positions <= k, inside the known part)."""
+    n = P.shape[0]
+    dev = s.device
+    Pc = ops.contiguous(P)
+    rows = ops.gemm(Pc.view(n, 1, n), Pc.view(n, n, 1)).view(1, n)          # squared row norms
+    ones = ops.fill_(torch.empty((n, 1), dtype=torch.float64, device=dev), 1.0)
+    tail = ops.gemm(rows, ones)                                               # ||P||_F^2, (1,1)
+    sv = ops.contiguous(s).view(1, -1)
+    s2 = ops.gemm(sv, ops.contiguous(sv.transpose(0, 1)))                     # sum s^2, (1,1)
+    one = ops.fill_(torch.empty((1, 1), dtype=torch.float64, device=dev), 1.0)
+    ops.gemm(one, s2, tail, alpha=-1.0, beta=1.0)
+    mag = ops.maxabs(tail)                                                    # |.| (rounding)
+    root = ops.fill_(torch.empty(1, dtype=torch.float64, device=dev), 1.0)
+    ops.scale_rows_(root, mag, 1, 1, 1)                                       # sqrt(tail)
+    ext = torch.empty(s.numel() + 1, dtype=torch.float64, device=dev)
     ops.copy_into(ext[:-1], s)
-    # a single trailing value carrying the discarded mass keeps the tail norms exact
-    tv = ops.contiguous(tail.abs() if False else tail)
-    ops.copy_into(ext[-1:], _sqrt_dev(tv))
+    ops.copy_into(ext[-1:], root)
     return linalg.truncate(ext, k, degeneracy_eps, truncate_eps)
-
-
-def _sqrt_dev(x):
-    y = ops.contiguous(x)
-    one = torch.empty_like(y)
-    ops.fill_(one, 1.0)
-    ops.scale_rows_(one, ops.contiguous(y), 1, 1, 1)   # one * sqrt(y)
-    return one
 
 
 def squeezer(A, UM, sM, VMH, D, Dcut):

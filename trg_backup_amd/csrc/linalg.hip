@@ -559,6 +559,8 @@ __global__ void __launch_bounds__(256) gesvj_kernel(int m, int n, const double *
 
 // Rank selection of utility/truncated_svd.py:17-78 on the device for one
 // descending vector s (length n):  writes chi and the tail-norm error.
+constexpr int TRUNC_MAX = 4096;   // longest spectrum the rank rule accepts
+
 __global__ void truncate_kernel(int n, const double *s, int k, double deg_eps, double trunc_eps, int *chi_out,
                                 double *err_out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -576,27 +578,23 @@ __global__ void truncate_kernel(int n, const double *s, int k, double deg_eps, d
     for (chi1 = k; chi1 > 0; --chi1)
         if (delta(chi1) == 1) break;
     if (chi1 == 0) chi1 = 1;
+    // suffix sums suf[i] = sum_{j>=i} (s_j/s_0)^2 (no cancellation for tiny tails)
+    __shared__ double suf[TRUNC_MAX + 1];
+    suf[n] = 0.0;
+    for (int i = n - 1; i >= 0; --i) suf[i] = suf[i + 1] + (s[i] / s0) * (s[i] / s0);
     double tot = 0.0;
-    for (int i = n - 1; i >= 0; --i) tot += (s[i] / s0) * (s[i] / s0);
-    // errors[i] = sqrt(sum_{j>=i} s_j^2 / tot), decreasing in i
+    for (int i = 0; i < n; ++i) tot += (s[i] / s0) * (s[i] / s0);
+    // errors[i] = sqrt(suf[i] / tot), decreasing in i
     int chi2 = 0;
-    double tail = tot;
-    for (int i = 0; i < chi1; ++i) {
-        if (sqrt(tail / tot) > trunc_eps) ++chi2;
-        tail -= (s[i] / s0) * (s[i] / s0);
-    }
+    for (int i = 0; i < chi1; ++i)
+        if (sqrt(suf[i] / tot) > trunc_eps) ++chi2;
     int chi = chi1;
     if (chi2 < chi1) {
         for (chi = chi2; chi <= chi1; ++chi)
             if (delta(chi) == 1) break;
         if (chi > chi1) chi = chi1;
     }
-    double err = 0.0;
-    if (chi < n) {
-        double t = 0.0;
-        for (int i = n - 1; i >= chi; --i) t += (s[i] / s0) * (s[i] / s0);
-        err = sqrt(t / tot);
-    }
+    const double err = chi < n ? sqrt(suf[chi] / tot) : 0.0;
     *chi_out = chi;
     *err_out = err;
 }
@@ -625,7 +623,7 @@ extern "C" int trg_truncate_rank(int n, const double *s, int k, double degenerac
                                  double *err, void *stream) {
     if (n < 0 || k < 0) return TRG_BAD_ARG;
     if (!s || !chi || !err) return TRG_BAD_ARG;
-    if (n == 0) return TRG_SHAPE;
+    if (n == 0 || n > TRUNC_MAX) return TRG_SHAPE;
     hipLaunchKernelGGL(truncate_kernel, 1, 64, 0, as_stream(stream), n, s, k, degeneracy_eps, truncate_eps, chi, err);
     return check_launch();
 }

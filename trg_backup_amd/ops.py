@@ -328,3 +328,18 @@ def identity(n, dtype, device, batch=1):
     diag = torch.as_strided(I, (batch, n), (n * n, n + 1))
     copy_into(diag, ones.view(batch, n))
     return I
+
+
+def flip(t, dim):
+    """Copy of t reversed along `dim` (a negative-stride read through libtrg)."""
+    require_device(t)
+    out = torch.empty(tuple(t.shape), dtype=t.dtype, device=t.device)
+    n = t.shape[dim]
+    if out.numel() == 0:
+        return out
+    si = list(t.stride())
+    base = ptr(t) + (n - 1) * si[dim] * t.element_size()
+    si[dim] = -si[dim]
+    call("trg_copy_strided", dtype_code(t), dtype_code(out), t.dim(), ll_array(list(t.shape)), base, ll_array(si),
+         ptr(out), ll_array(list(out.stride())), stream_ptr())
+    return out
