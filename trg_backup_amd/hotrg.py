@@ -25,8 +25,10 @@ from .ops import gemm, permute
 
 # subspace-iteration settings of the isometry eigensolver
 EIG = {"l_factor": 2, "q": 12, "seed": 20240}
-# memory budget (bytes) for the chi^5 / chi^8 merge intermediates of one batch
-MERGE_BUDGET = 24 << 30
+# memory budgets (bytes) for the chi^5 / chi^8 merge intermediates: per
+# instance (sets the chunk length, a function of the shape only) and per batch
+INSTANCE_BUDGET = 1 << 30
+MERGE_BUDGET = 32 << 30
 
 
 @dataclass(frozen=True)
@@ -161,8 +163,19 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
     B, Dx, _, Dy, _ = T1.shape
     chi_a, chi_b = Ua.shape[3], Ub.shape[3]
     es = T1.element_size()
-    per_c = 2 * Dy * Dy * Dx * max(chi_a, Dy) * es * B
-    cc = max(1, min(Dx, MERGE_BUDGET // max(per_c, 1)))
+    # The chunk length depends on the per-instance shape only (never on B),
+    # so an instance's summation order is the same alone and in any batch;
+    # batches too large for the budget are split into sub-batches instead.
+    per_c = 2 * Dy * Dy * Dx * max(chi_a, Dy) * es
+    cc = max(1, min(Dx, INSTANCE_BUDGET // max(per_c, 1)))
+    nsub = max(1, min(B, MERGE_BUDGET // max(per_c * cc, 1)))
+    if B > nsub:
+        outs = [merge2d(T1[b0:b0 + nsub], T2[b0:b0 + nsub], Ua[b0:b0 + nsub], Ub[b0:b0 + nsub], compute)
+                for b0 in range(0, B, nsub)]
+        res = torch.empty((B,) + tuple(outs[0].shape[1:]), dtype=T1.dtype, device=T1.device)
+        for i, b0 in enumerate(range(0, B, nsub)):
+            ops.copy_into(res[b0:b0 + outs[i].shape[0]], outs[i])
+        return res
     Tp = permute(T2, (0, 3, 1, 2, 4)).view(B, Dy * Dx, Dx * Dy)          # [(e,b),(d,l)]
     T1m = T1.reshape(B, Dx, Dx * Dy * Dy)                                 # [a,(c,k,e)]
     Uam = Ua.reshape(B, Dx, Dx * chi_a)                                   # [a,(b,i)]
@@ -306,8 +319,16 @@ def merge3d(T1, T2, U1a, U1b, U2a, U2b, compute=None):
     ca, cb = U1a.shape[3], U1b.shape[3]
     cc_, cd = U2a.shape[3], U2b.shape[3]
     es = T1.element_size()
-    per_a = 2 * Dx * Dy * Dz * cc_ * Dx * Dy * Dz * es * B     # X2/X3 size per unit of a
-    na = max(1, min(ca, MERGE_BUDGET // max(per_a, 1)))
+    per_a = 2 * Dx * Dy * Dz * cc_ * Dx * Dy * Dz * es     # X2/X3 bytes per unit of a, per instance
+    na = max(1, min(ca, INSTANCE_BUDGET // max(per_a, 1)))   # shape-only chunk length (batch invariant)
+    nsub = max(1, min(B, MERGE_BUDGET // max(per_a * na, 1)))
+    if B > nsub:
+        outs = [merge3d(T1[b0:b0 + nsub], T2[b0:b0 + nsub], U1a[b0:b0 + nsub], U1b[b0:b0 + nsub],
+                        U2a[b0:b0 + nsub], U2b[b0:b0 + nsub], compute) for b0 in range(0, B, nsub)]
+        res = torch.empty((B,) + tuple(outs[0].shape[1:]), dtype=T1.dtype, device=T1.device)
+        for i, b0 in enumerate(range(0, B, nsub)):
+            ops.copy_into(res[b0:b0 + outs[i].shape[0]], outs[i])
+        return res
     out = torch.empty((B, ca, cb, cc_, cd, Dz, Dz), dtype=T1.dtype, device=T1.device)
     # T2 as [(z,m,o),(n,p,f)]: permute legs (m,n,o,p,z,f) -> (z,m,o,n,p,f)
     T2p = permute(T2, (0, 5, 1, 3, 2, 4, 6)).view(B, Dz * Dx * Dy, Dx * Dy * Dz)

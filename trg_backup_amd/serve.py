@@ -1,0 +1,260 @@
+"""Continuous-batching server for phase-diagram solves (mixed model, chi, beta).
+
+A request asks for ln Z/V of one model at one coupling, bond dimension and
+number of RG steps.  The server
+
+  * admits queued requests in arrival order while the sum of the HBM
+    footprints of the admitted flows stays within the budget (requests whose
+    footprint alone exceeds the budget are rejected);
+  * keeps every admitted flow's tensors resident on the GPU between steps;
+  * advances all active flows by one RG step per tick: 2D HOTRG flows that
+    are at the same step shape and precision are stacked into one batched
+    launch sequence; ATRG flows (dynamic bonds) step individually; the groups
+    of a tick run concurrently on a small pool of HIP streams (one host
+    thread each), so the work of different requests overlaps on the device;
+  * admits new arrivals at every tick, as memory and slots free up;
+  * records the latency arrival -> result of every request.
+
+Every kernel is batch-invariant (an instance's arithmetic does not depend on
+which other instances share its launch), so a request's result is the same
+alone and in the mix; tests/test_gpu_serve.py checks that.
+
+Model -> scheme: 2D Ising runs HOTRG (trg/HOTRG_2d.py); 3D Ising and 3D q=3
+Potts run ATRG (trg/ATRG_3d_new.py, the reference's RGSCHEME=atrg path,
+measurement/Ising_model_3d.py:114-122, measurement/Potts3_model_3d.py:32-40),
+because 3D HOTRG needs chi^6 storage and 2 chi^11 flop per step (275 GB and
+1.5e20 flop at chi = 64).
+"""
+import itertools
+import math
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import atrg, hotrg, models, ops
+
+
+@dataclass
+class Request:
+    rid: int
+    model: str            # "ising2d" | "ising3d" | "potts3d"
+    chi: int
+    beta: float
+    steps: int
+    arrival: float = 0.0  # seconds after the start of the stream
+
+
+@dataclass
+class Flow:
+    req: Request
+    scheme: str
+    footprint: int
+    step: int = 0
+    T: object = None           # 2D HOTRG tensor (1, D, D, D, D)
+    lnf: object = None         # (1, steps + 1) float64
+    U: object = None
+    s: object = None
+    VH: object = None
+    consts: list = field(default_factory=list)
+    admitted_at: float = 0.0
+    result: float = None
+    finished_at: float = None
+
+
+def footprint(req, policy=None):
+    """HBM bytes a flow holds at its peak step (tensor, environments,
+    eigensolver and merge / coarse-graining intermediates)."""
+    chi = req.chi
+    if req.model == "ising2d":
+        t = chi ** 4 * 8
+        env = 2 * chi ** 4 * 8 * 3
+        per_c = 2 * chi ** 4 * chi * 4
+        merge = 4 * per_c * max(1, min(chi, hotrg.INSTANCE_BUDGET // max(per_c, 1)))
+        return int(3 * t + env + merge)
+    # ATRG: U, VH, G, H (chi^4 each), squeezer environments and roots (chi^4 float64 each),
+    # range-finder blocks (chi^3 x 2 chi)
+    return int(16 * chi ** 4 * 8 + 6 * chi ** 3 * 2 * chi * 8 + chi ** 5 * 8)
+
+
+class Server:
+    def __init__(self, hbm_budget=None, max_batch=32, streams=2, fp64_steps=10, atrg_q=2, atrg_deg_eps=1e-8):
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info()
+            hbm_budget = int(0.6 * free)
+        self.budget = hbm_budget
+        self.max_batch = max_batch
+        self.policy = hotrg.Policy(fp64_steps=fp64_steps, low=hotrg.FP32)
+        self.atrg_q = atrg_q
+        self.atrg_deg_eps = atrg_deg_eps
+        self.streams = [torch.cuda.Stream() for _ in range(streams)]   # default priority
+        self.pool = ThreadPoolExecutor(max_workers=streams)
+        self.lock = threading.Lock()
+        self.rejected = []
+        self.max_used = 0
+
+    # ---- flow set-up and stepping ----------------------------------------
+    def _start(self, fl):
+        r = fl.req
+        if fl.scheme == "hotrg2d":
+            T = models.initial_tensor(r.model, r.beta)[None].contiguous()
+            fl.lnf = ops.fill_(torch.empty((1, r.steps + 1), dtype=torch.float64, device=T.device), 0.0)
+            hotrg.normalize_maxabs(T, fl.lnf, 0)
+            fl.T = T
+        else:
+            U, s, VH = models.atrg_initial(r.model, r.beta)
+            s, c = atrg.normalize(U, s, VH)
+            fl.U, fl.s, fl.VH = U, s, VH
+            fl.consts = [c]
+
+    def _hotrg_direction(self, step):
+        return "Y" if step % 2 == 1 else "X"       # trg/HOTRG_2d.py:195-214 (Y first)
+
+    def _atrg_direction(self, step):
+        return "TXY"[(step - 1) % 3]                # trg/ATRG_3d_new.py:999-1000
+
+    def _step_group(self, flows, stream):
+        with torch.cuda.stream(stream):
+            if flows[0].scheme == "hotrg2d":
+                n0 = flows[0].step + 1
+                prec = self.policy(n0)
+                if len(flows) > 1:
+                    T = torch.empty((len(flows),) + tuple(flows[0].T.shape[1:]), dtype=flows[0].T.dtype,
+                                    device=flows[0].T.device)
+                    for i, f in enumerate(flows):
+                        ops.copy_into(T[i:i + 1], f.T)
+                else:
+                    T = flows[0].T
+                T = hotrg.step2d(T, flows[0].req.chi, self._hotrg_direction(n0), prec)
+                c = ops.maxabs(T, batch=T.shape[0])
+                for i, f in enumerate(flows):
+                    n = f.step + 1
+                    Ti = ops.contiguous(T[i:i + 1])
+                    ops.normalize_(Ti, c[i:i + 1], f.lnf, n, batch=1, lnf_stride=f.lnf.shape[1])
+                    f.T = Ti
+                    f.step = n
+            else:
+                old = atrg.degeneracy_eps
+                for f in flows:
+                    n = f.step + 1
+                    U, s, VH = atrg.atrg_pure_tensor(f.U, f.s, f.VH, f.req.chi, self._atrg_direction(n), "f64")
+                    s, c = atrg.normalize(U, s, VH)
+                    f.U, f.s, f.VH = U, s, VH
+                    f.consts.append(c)
+                    f.step = n
+            stream.synchronize()
+
+    def _finish(self, fl, now):
+        r = fl.req
+        if fl.scheme == "hotrg2d":
+            fl.result = float(hotrg.lnz2d(fl.T, fl.lnf, r.steps)[0].item())
+            fl.T = fl.lnf = None
+        else:
+            tr = atrg.trace(fl.U, fl.s, fl.VH)
+            lg = sum(math.log(abs(float(c[0].item()))) / 2 ** k for k, c in enumerate(fl.consts))
+            fl.result = lg + math.log(abs(float(tr[0].item()))) / 2 ** r.steps
+            fl.U = fl.s = fl.VH = None
+        fl.finished_at = now
+
+    # ---- the loop -----------------------------------------------------------
+    def run(self, requests, realtime=True, log=None):
+        """Serve `requests` (arrival times in seconds).  Returns the finished flows."""
+        atrg_params = dict(atrg.RSVD)
+        atrg.set_rsvd_params(k_factor=2, p=0, q=self.atrg_q, seed=1)
+        old_deg = atrg.degeneracy_eps
+        atrg.degeneracy_eps = self.atrg_deg_eps
+        try:
+            return self._run(sorted(requests, key=lambda r: (r.arrival, r.rid)), realtime, log)
+        finally:
+            atrg.RSVD.update(atrg_params)
+            atrg.degeneracy_eps = old_deg
+
+    def _run(self, queue, realtime, log):
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        active, done, used = [], [], 0
+        qi = 0
+        while qi < len(queue) or active:
+            # admission in arrival order, by HBM footprint
+            while qi < len(queue) and (not realtime or queue[qi].arrival <= now()):
+                r = queue[qi]
+                scheme = "hotrg2d" if r.model == "ising2d" else "atrg3d"
+                fp = footprint(r)
+                if fp > self.budget:
+                    self.rejected.append(r)
+                    qi += 1
+                    continue
+                if used + fp > self.budget:
+                    break
+                fl = Flow(req=r, scheme=scheme, footprint=fp, admitted_at=now())
+                self._start(fl)
+                active.append(fl)
+                used += fp
+                self.max_used = max(self.max_used, used)
+                qi += 1
+            if not active:
+                if qi < len(queue) and realtime:
+                    time.sleep(max(0.0, min(0.01, queue[qi].arrival - now())))
+                continue
+            # set-up work of newly admitted flows ran on this thread's stream;
+            # the worker streams must see it complete
+            torch.cuda.current_stream().synchronize()
+            # one RG step for every active flow: group 2D flows by step shape and precision
+            groups = {}
+            for f in active:
+                if f.scheme == "hotrg2d":
+                    n = f.step + 1
+                    key = ("h", f.req.chi, tuple(f.T.shape[1:]), f.T.dtype, self._hotrg_direction(n), self.policy(n))
+                else:
+                    key = ("a", f.req.rid)
+                groups.setdefault(key, []).append(f)
+            batches = []
+            for key, fl in groups.items():
+                for i in range(0, len(fl), self.max_batch):
+                    batches.append(fl[i:i + self.max_batch])
+            futs = [self.pool.submit(self._step_group, b, self.streams[i % len(self.streams)])
+                    for i, b in enumerate(batches)]
+            for fu in futs:
+                fu.result()
+            still = []
+            for f in active:
+                if f.step >= f.req.steps:
+                    self._finish(f, now())
+                    used -= f.footprint
+                    done.append(f)
+                    if log:
+                        log(f"done rid={f.req.rid} {f.req.model} chi={f.req.chi} steps={f.req.steps} "
+                            f"lnZ/V={f.result:.12f} latency={f.finished_at - f.req.arrival:.3f}s")
+                else:
+                    still.append(f)
+            active = still
+        return done
+
+
+def poisson_stream(n, rate, seed, chis=(16, 24, 32, 48, 64), steps=(20, 40),
+                   models_=("ising2d", "ising3d", "potts3d"), window=0.1):
+    """The serving workload: Poisson arrivals (exponential gaps, `rate` per
+    second), model uniform over `models_`, chi uniform over `chis`, beta
+    uniform in [1 - window, 1 + window] x beta_c, steps uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        m = models_[rng.integers(len(models_))]
+        chi = int(chis[rng.integers(len(chis))])
+        b = models.BETA_C[m] * rng.uniform(1 - window, 1 + window)
+        st = int(rng.integers(steps[0], steps[1] + 1))
+        out.append(Request(rid=i, model=m, chi=chi, beta=float(b), steps=st, arrival=float(t)))
+    return out
+
+
+def summarize(done, t_total):
+    lat = np.array([f.finished_at - f.req.arrival for f in done])
+    return {"requests": len(done), "solves_per_s": len(done) / t_total if t_total > 0 else None,
+            "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+            "rg_steps": int(sum(f.req.steps for f in done))}
