@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-n", type=int, default=16)
     return ap.parse_args()
 
 
@@ -167,6 +169,27 @@ def main():
         del Td
         out["lnZ_err_vs_fp64"] = float(np.max(np.abs(zm - zd)))
         out["lnZ_check"] = {"betas": vb, "mixed": zm.tolist(), "fp64": zd.tolist()}
+        torch.cuda.empty_cache()
+
+    # ---- serving: a short Poisson stream of mixed solves (latency) ----
+    if not args.no_serve:
+        from trg_backup_amd import serve
+        reqs = serve.poisson_stream(args.serve_n, 50.0, 2, chis=(16, 24, 32), steps=(20, 30))
+        srv = serve.Server(streams=2)
+        t1 = time.perf_counter()
+        done = srv.run(reqs, realtime=True)
+        ts = time.perf_counter() - t1
+        summ = serve.summarize(done, ts)
+        # float64 references of the 2D (mixed-precision) solves; ATRG flows run float64 throughout
+        twod = [f.req for f in done if f.req.model == "ising2d"]
+        ref = {f.req.rid: f.result for f in serve.Server(fp64_steps=10 ** 9).run(twod, realtime=False)} if twod else {}
+        err = max([abs(f.result - ref[f.req.rid]) for f in done if f.req.rid in ref] or [0.0])
+        out["serving"] = dict(summ, worst_lnZ_err_vs_fp64=err, rejected=len(srv.rejected),
+                              stream=f"{args.serve_n} Poisson requests (rate 50/s, seed 2), models "
+                                     "{2D Ising HOTRG, 3D Ising ATRG, 3D Potts ATRG}, chi in {16,24,32}, "
+                                     "20-30 RG steps, beta within 10% of beta_c")
+        out["p50_latency_s"] = summ["p50_latency_s"]
+        out["p99_latency_s"] = summ["p99_latency_s"]
         torch.cuda.empty_cache()
 
     # ---- CPU baseline: the NumPy float64 oracle on the host ----

@@ -36,6 +36,21 @@ static int lds_limit() {
     return g_lds_limit[dev];
 }
 
+// Raise a kernel's dynamic-LDS limit to the device maximum once per process
+// and device, so that later launches (including ones recorded into a HIP
+// graph) issue no attribute calls.  Returns non-zero on failure.
+static int ensure_dyn_lds(const void *fn, int slot) {
+    static std::once_flag once[4][64];
+    static int status[4][64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || slot < 0 || slot >= 4) return 1;
+    std::call_once(once[slot][dev], [&]() {
+        const int lim = lds_limit() - 16384;
+        status[slot][dev] = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lim) == hipSuccess ? 0 : 1;
+    });
+    return status[slot][dev];
+}
+
 // ---------------------------------------------------------------------------
 // Jacobi eigensolver.  Work matrix W (n x n, row-major) either in LDS or in
 // the caller's global buffer.  V accumulates the rotations.
@@ -318,8 +333,7 @@ extern "C" int trg_syevj_f64(int batch, int n, double *A, long long sA, double *
     const size_t bytes = (size_t)n * n * sizeof(double);
     const int use_lds = (bytes + 16384 <= (size_t)lds_limit()) ? 1 : 0;
     if (use_lds && bytes > 65536) {
-        if (hipFuncSetAttribute((const void *)syevj_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
-            hipSuccess) {
+        if (ensure_dyn_lds((const void *)syevj_kernel, 0)) {
             g_last_hip_error = (int)hipGetLastError();
             return TRG_HIP_ERROR;
         }
@@ -337,10 +351,9 @@ extern "C" int trg_chol_inv_f64(int batch, int l, double *G, long long sG, doubl
     if (!G || !Rinv) return TRG_BAD_ARG;
     if (batch > 65535) return TRG_SHAPE;
     const size_t bytes = (size_t)l * l * sizeof(double);
-    const int use_lds = (bytes + 8192 <= (size_t)lds_limit()) ? 1 : 0;
+    const int use_lds = (bytes + 16384 <= (size_t)lds_limit()) ? 1 : 0;
     if (use_lds && bytes > 65536) {
-        if (hipFuncSetAttribute((const void *)chol_inv_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
-            hipSuccess) {
+        if (ensure_dyn_lds((const void *)chol_inv_kernel, 1)) {
             g_last_hip_error = (int)hipGetLastError();
             return TRG_HIP_ERROR;
         }
@@ -609,8 +622,7 @@ extern "C" int trg_gesvj_f64(int batch, int m, int n, const double *A, long long
     const size_t bytes = (size_t)m * n * sizeof(double);
     if (bytes + 16384 > (size_t)lds_limit()) return TRG_SHAPE;
     if (bytes > 65536) {
-        if (hipFuncSetAttribute((const void *)gesvj_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
-            hipSuccess) {
+        if (ensure_dyn_lds((const void *)gesvj_kernel, 2)) {
             g_last_hip_error = (int)hipGetLastError();
             return TRG_HIP_ERROR;
         }
@@ -636,10 +648,9 @@ extern "C" int trg_potrf_psd_f64(int batch, int l, double *G, long long sG, doub
     if (!G || !Rinv) return TRG_BAD_ARG;
     if (batch > 65535) return TRG_SHAPE;
     const size_t bytes = (size_t)l * l * sizeof(double);
-    const int use_lds = (bytes + 8192 <= (size_t)lds_limit()) ? 1 : 0;
+    const int use_lds = (bytes + 16384 <= (size_t)lds_limit()) ? 1 : 0;
     if (use_lds && bytes > 65536) {
-        if (hipFuncSetAttribute((const void *)chol_inv_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
-            hipSuccess) {
+        if (ensure_dyn_lds((const void *)chol_inv_kernel, 1)) {
             g_last_hip_error = (int)hipGetLastError();
             return TRG_HIP_ERROR;
         }

@@ -1,0 +1,62 @@
+"""HIP-graph capture of steady-state batched HOTRG steps.
+
+Once the bonds stop growing, every Y (and every X) step of a batch has the
+same shapes, so the whole step -- environments, batched top-k eigensolver,
+on-device LD/RU choice, merge, max-|T| normalisation and the log-norm write
+-- is recorded once into a HIP graph and replayed.  The step contains no host
+synchronisation and no memset: flags and counters are cleared by fill
+kernels, and the log-norm slot comes from a device-side step counter that the
+graph itself advances (trg_normalize with step_ptr).
+
+    step = device int32 counter shared by the Y and X graphs
+    gY = GraphedStep2D(batch, chi, "Y", prec, lnf, step)
+    gX = GraphedStep2D(batch, chi, "X", prec, lnf, step)
+    set_step(step, n); T = gY(T); T = gX(T); ...
+"""
+import torch
+
+from . import hotrg, ops
+
+
+def step_counter(device, value=0):
+    s = torch.empty(1, dtype=torch.int32, device=device)
+    ops.fill_(s, value)
+    return s
+
+
+def set_step(counter, n):
+    ops.fill_(counter, int(n))
+
+
+class GraphedStep2D:
+    def __init__(self, batch, chi, direction, prec, lnf, step, warmup=1):
+        dev = lnf.device
+        self.batch, self.chi, self.direction, self.prec = batch, chi, direction, prec
+        self.inp = torch.empty((batch, chi, chi, chi, chi), dtype=prec.store, device=dev)
+        ops.fill_(self.inp, 1.0)
+        # warm-up on a side stream writes into scratch log-norm / counter buffers
+        scratch_lnf = ops.fill_(torch.empty(lnf.shape, dtype=torch.float64, device=dev), 0.0)
+        scratch_step = step_counter(dev)
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                self._body(scratch_lnf, scratch_step)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = self._body(lnf, step)
+        torch.cuda.synchronize()
+
+    def _body(self, lnf, step):
+        T = hotrg.step2d(self.inp, self.chi, self.direction, self.prec)
+        c = ops.maxabs(T, batch=self.batch)
+        ops.normalize_(T, c, lnf, 0, batch=self.batch, lnf_stride=lnf.shape[1], step_ptr=step)
+        return T
+
+    def __call__(self, T):
+        if T.data_ptr() != self.inp.data_ptr():
+            ops.copy_into(self.inp, T)
+        self.graph.replay()
+        return self.out
