@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-n", type=int, default=16)
+    ap.add_argument("--no-tune", action="store_true")
     return ap.parse_args()
 
 
@@ -57,8 +58,7 @@ def main():
     import numpy as np
     import torch
 
-    from oracle import models
-    from trg_backup_amd import _lib, hotrg, ops
+    from trg_backup_amd import _lib, autotune, hotrg, models, ops
 
     _lib.load()
     dev = torch.device("cuda:0")
@@ -68,7 +68,7 @@ def main():
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
 
     # ---- growth to the steady shape (untimed) ----
-    T = torch.stack([torch.as_tensor(models.ising2d_tensor(b)) for b in betas]).to(dev)
+    T = torch.stack([models.initial_tensor("ising2d", b, dev) for b in betas])
     nsteps_total = 6 + args.warmup + args.steps
     lnf = ops.fill_(torch.empty((nb, nsteps_total + 1), dtype=torch.float64, device=dev), 0.0)
     T = ops.contiguous(T)
@@ -89,11 +89,14 @@ def main():
         hotrg.normalize_maxabs(T, lnf, st)
         return T
 
+    # warm-up steps also pick the GEMM tile configurations (tuning is untimed)
+    autotune.enable(not args.no_tune)
     for _ in range(args.warmup):
         step += 1
         T = rg_step(T, direction, step)
         direction = "X" if direction == "Y" else "Y"
     torch.cuda.synchronize()
+    autotune.enable(False)
 
     rec = ops.GemmRecorder("merge2d_z2")
     t0 = time.perf_counter()
@@ -125,8 +128,9 @@ def main():
                    "chi": chi, "nbeta": nb, "policy": policy.describe(), "parallelism": "single GPU, batched"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "trg::gemm_kernel<float,128,128,32,64,64,true,true,4> (merge chi^7 GEMM, "
-                               "M=nc*chi^2 N=chi^2 K=chi^2 batch=nbeta)",
+                     "kernel": "trg::gemm_kernel<float,...> tiles "
+                               + ", ".join(sorted({autotune.config_name(a, b, c) for a, b, c, *_ in rec.configs}))
+                               + " (merge chi^7 GEMM, M=nc*chi^2 N=chi^2 K=chi^2 batch=nbeta)",
                      "launches": g["launches"], "avg_ms": g["avg_ms"], "flops_per_launch": g["avg_flops"]},
     }
     del T
@@ -134,7 +138,7 @@ def main():
 
     # ---- per-instance float64 path with a host sync per step ----
     if not args.no_plain:
-        T1 = torch.as_tensor(models.ising2d_tensor(0.44)).to(dev)[None].contiguous()
+        T1 = models.initial_tensor("ising2d", 0.44, dev)[None].contiguous()
         l1 = ops.fill_(torch.empty((1, 64), dtype=torch.float64, device=dev), 0.0)
         s, d = 0, "Y"
         while T1.shape[1] < chi or T1.shape[3] < chi:
@@ -160,7 +164,7 @@ def main():
     # ---- accuracy of the mixed policy against the float64 path (untimed) ----
     if not args.no_validate:
         vb = [0.40, 0.44]
-        T0 = torch.stack([torch.as_tensor(models.ising2d_tensor(b)) for b in vb]).to(dev)
+        T0 = torch.stack([models.initial_tensor("ising2d", b, dev) for b in vb])
         Tm, lm = hotrg.renorm2d(T0, chi, 20, 20, policy)
         zm = hotrg.lnz2d(Tm, lm, 40).cpu().numpy()
         del Tm
@@ -195,7 +199,8 @@ def main():
     # ---- CPU baseline: the NumPy float64 oracle on the host ----
     if not args.no_cpu:
         from oracle import HOTRG_2d as oh2
-        Tc = models.ising2d_tensor(0.44)
+        from oracle import models as omodels
+        Tc = omodels.ising2d_tensor(0.44)
         s, d = 0, "Y"
         while Tc.shape[0] < chi or Tc.shape[2] < chi:
             s += 1

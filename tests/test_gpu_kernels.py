@@ -157,3 +157,60 @@ def test_permute_fast_paths(gpu, perm):
     xs = xd[:, 3:30, ::2, :]
     ys = ops.permute(xs, perm)
     assert np.array_equal(ys.cpu().numpy(), np.transpose(x[:, 3:30, ::2, :], perm))
+
+
+@pytest.mark.parametrize("dtype,compute", [(torch.float32, None), (torch.float32, "bf16"), (torch.float64, None)])
+@pytest.mark.parametrize("shape,ta,tb", [((300, 260, 100), False, False), ((257, 129, 67), True, False),
+                                         ((96, 200, 48), False, True), ((64, 4096, 64), True, True),
+                                         ((300, 260, 100), False, True), ((130, 70, 67), False, True),
+                                         ((520, 264, 1024), False, True)])
+def test_gemm_tile_configs_bitwise_equal(gpu, dtype, compute, shape, ta, tb):
+    """Every tile configuration gives the same bits as the default dispatch
+    (same MFMA instruction, same K order) and agrees with a float64 product."""
+    from trg_backup_amd import _lib, autotune, ops
+    M, N, K = shape
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((3, K, M) if ta else (3, M, K))
+    B = rng.standard_normal((3, N, K) if tb else (3, K, N))
+    Ad, Bd = _dev(A, gpu).to(dtype), _dev(B, gpu).to(dtype)
+    Av = Ad.transpose(1, 2) if ta else Ad
+    Bv = Bd.transpose(1, 2) if tb else Bd
+    C0 = ops.gemm(Av, Bv, compute=compute)
+    ref = np.matmul(A.transpose(0, 2, 1) if ta else A, B.transpose(0, 2, 1) if tb else B)
+    err = np.abs(C0.double().cpu().numpy() - ref).max() / (np.abs(A).max() * np.abs(B).max() * K)
+    assert err < (2e-3 if compute == "bf16" else 1e-6 if dtype == torch.float32 else 1e-14), err
+    dt = ops.dtype_code(Ad)
+    cc = _lib.BF16 if compute == "bf16" else dt
+    n = autotune.config_count(dt, cc)
+    assert n >= 2
+    for cfg in range(n):
+        C = torch.empty_like(C0)
+        _lib.call("trg_gemm_ex", cfg, dt, cc, 3, M, N, K, 1.0, Av.data_ptr(), Av.stride(0), Av.stride(1),
+                  Av.stride(2), Bv.data_ptr(), Bv.stride(0), Bv.stride(1), Bv.stride(2), 0.0, C.data_ptr(),
+                  M * N, N, 1, ops.stream_ptr())
+        assert torch.equal(C, C0), (cfg, autotune.config_name(dt, cc, cfg))
+
+
+def test_gemm_autotuner_picks_a_config_and_keeps_bits(gpu, tmp_path):
+    from trg_backup_amd import autotune, ops
+    rng = np.random.default_rng(5)
+    A = _dev(rng.standard_normal((4, 512, 256)), gpu).float()
+    B = _dev(rng.standard_normal((4, 256, 384)), gpu).float()
+    C0 = ops.gemm(A, B)
+    autotune.clear()
+    autotune.enable(True, reps=2)
+    try:
+        C1 = ops.gemm(A, B)
+        C2 = ops.gemm(A, B)                      # table hit
+    finally:
+        autotune.enable(False)
+    tab = autotune.table()
+    assert len(tab) == 1 and 0 <= next(iter(tab.values())) < autotune.config_count(0, 0)
+    assert torch.equal(C1, C0) and torch.equal(C2, C0)
+    p = tmp_path / "tune.json"
+    autotune.save(str(p))
+    autotune.clear()
+    autotune.load(str(p))
+    assert autotune.table() == tab and autotune.active()
+    assert torch.equal(ops.gemm(A, B), C0)
+    autotune.clear()

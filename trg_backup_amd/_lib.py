@@ -25,6 +25,11 @@ SIGNATURES = {
     "trg_gemm": [c_int, c_int, c_int, c_int, c_int, c_int, c_dbl,
                  c_vp, c_ll, c_ll, c_ll, c_vp, c_ll, c_ll, c_ll,
                  c_dbl, c_vp, c_ll, c_ll, c_ll, c_vp],
+    "trg_gemm_ex": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_dbl,
+                    c_vp, c_ll, c_ll, c_ll, c_vp, c_ll, c_ll, c_ll,
+                    c_dbl, c_vp, c_ll, c_ll, c_ll, c_vp],
+    "trg_gemm_config_count": [c_int, c_int],
+    "trg_gemm_config_name": [c_int, c_int, c_int],
     "trg_copy_strided": [c_int, c_int, c_int, P_LL, c_vp, P_LL, c_vp, P_LL, c_vp],
     "trg_fill": [c_int, c_ll, c_dbl, c_vp, c_vp],
     "trg_maxabs": [c_int, c_int, c_ll, c_vp, c_ll, c_vp, c_vp, c_int, c_vp],
@@ -47,6 +52,7 @@ SIGNATURES = {
     "trg_axpy_dev_f64": [c_int, c_ll, c_dbl, c_vp, c_vp, c_ll, c_vp, c_ll, c_vp],
 }
 _RESTYPES = {"trg_version": ctypes.c_char_p, "trg_status_string": ctypes.c_char_p,
+             "trg_gemm_config_name": ctypes.c_char_p,
              "trg_eigh_topk_workspace": c_ll, "trg_orthonormalize_workspace": c_ll}
 
 

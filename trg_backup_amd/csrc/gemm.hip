@@ -376,6 +376,139 @@ gemm_bf16_kernel(GemmArgs g) {
 }
 
 // ---------------------------------------------------------------------------
+// fp32 storage, bf16 MFMA, both operands K-contiguous: A(m,k) at m*rsA + k and
+// B(k,n) at n*csB + k.  Each thread fetches 16-byte fp32 vectors along k,
+// rounds them to bf16 and writes 8-byte runs into the k-fastest LDS rows that
+// the MFMA operand fetch (8 consecutive k per lane, one 16-byte LDS read)
+// expects, so no transposition happens on the way.  The row pitch BK + 8
+// keeps the 16-byte reads of 16 consecutive rows on disjoint LDS banks.
+// Requires 16-byte aligned bases, and K, rsA, csB, sAb, sBb multiples of 4
+// (a vector is then either fully inside or fully outside K).  Same MFMA
+// instruction and k order as gemm_bf16_kernel, so the two agree bitwise.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
+gemm_bf16k_kernel(GemmArgs g) {
+    constexpr int WAVES_M = BM / WM;
+    constexpr int NT = (BM / WM) * (BN / WN) * 64;
+    constexpr int TM = WM / 32;
+    constexpr int TN = WN / 32;
+    constexpr int KP = BK + 8;
+    constexpr int KQ = BK / 4;            // fp32 vectors per tile row
+    constexpr int AV = BM * KQ / NT;
+    constexpr int BV = BN * KQ / NT;
+    static_assert(AV * NT == BM * KQ && BV * NT == BN * KQ, "tile not divisible by threads");
+    static_assert(BK % 16 == 0, "BK must be a multiple of 16");
+
+    __shared__ __attribute__((aligned(16))) __bf16 As[BM * KP];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[BN * KP];
+
+    int tm, tn;
+    tile_coords(g, tm, tn);
+    const int b = blockIdx.y;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const float *__restrict__ A = static_cast<const float *>(g.A) + (long long)b * g.sAb;
+    const float *__restrict__ B = static_cast<const float *>(g.B) + (long long)b * g.sBb;
+    float *__restrict__ C = static_cast<float *>(g.C) + (long long)b * g.sCb;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    f32x4 ra[AV], rb[BV];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < AV; ++i) {
+            const int e = tid + i * NT;
+            const int m = m0 + e / KQ, k = k0 + 4 * (e % KQ);
+            if (m < g.M && k < g.K) ra[i] = *reinterpret_cast<const f32x4 *>(A + (long long)m * g.rsA + k);
+            else ra[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int i = 0; i < BV; ++i) {
+            const int e = tid + i * NT;
+            const int n = n0 + e / KQ, k = k0 + 4 * (e % KQ);
+            if (n < g.N && k < g.K) rb[i] = *reinterpret_cast<const f32x4 *>(B + (long long)n * g.csB + k);
+            else rb[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int i = 0; i < AV; ++i) {
+            const int e = tid + i * NT;
+            bf16x4 v;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[t] = (__bf16)ra[i][t];
+            *reinterpret_cast<bf16x4 *>(&As[(e / KQ) * KP + 4 * (e % KQ)]) = v;
+        }
+#pragma unroll
+        for (int i = 0; i < BV; ++i) {
+            const int e = tid + i * NT;
+            bf16x4 v;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[t] = (__bf16)rb[i][t];
+            *reinterpret_cast<bf16x4 *>(&Bs[(e / KQ) * KP + 4 * (e % KQ)]) = v;
+        }
+    };
+
+    load(0);
+    store();
+    __syncthreads();
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        const bool more = k0 + BK < g.K;
+        if (more) load(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            bf16x8 a[TM], bb[TN];
+            const int kr = kk + 8 * (lane >> 5);
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+                a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm * WM + i * 32 + (lane & 31)) * KP + kr]);
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                bb[j] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn * WN + j * 32 + (lane & 31)) * KP + kr]);
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            store();
+            __syncthreads();
+        }
+    }
+
+    const float alpha = (float)g.alpha, beta = (float)g.beta;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                const int n = n0 + wn * WN + j * 32 + (lane & 31);
+                if (m < g.M && n < g.N) {
+                    const long long idx = (long long)m * g.rsC + (long long)n * g.csC;
+                    float v = alpha * acc[i][j][r];
+                    if (g.beta != 0.0) v += beta * C[idx];
+                    C[idx] = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
 // Host dispatch.
 // ---------------------------------------------------------------------------
 template <typename T, int BM, int BN, int BK, int WM, int WN, int VEC>
@@ -409,22 +542,98 @@ static int launch_t(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
     return launch_v<T, BM, BN, BK, WM, WN, 1>(g, batch, amc, bnc, s);
 }
 
-template <int BM, int BN, int BK, int WM, int WN>
+// Both operands K-contiguous with 16-byte vectors that never straddle K.
+static bool kvec_ok(const GemmArgs &g, bool amc, bool bnc) {
+    if (amc || bnc) return false;
+    auto al = [](const void *p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
+    return al(g.A) && al(g.B) && g.K % 4 == 0 && g.rsA % 4 == 0 && g.csB % 4 == 0 && g.sAb % 4 == 0 &&
+           g.sBb % 4 == 0;
+}
+
+// KONLY: the tile exists for the K-contiguous kernel only (its generic
+// counterpart would spill registers); other layouts run tile 256x128x32 w64x64,
+// which gives the same bits.
+template <int BM, int BN, int BK, int WM, int WN, bool KONLY = false>
 static int launch_bf16(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
+    if constexpr (KONLY) {
+        if (!kvec_ok(g, amc, bnc)) return launch_bf16<256, 128, 32, 64, 64>(g, batch, amc, bnc, s);
+    }
     g.tiles_m = (int)cdiv(g.M, BM);
     g.tiles_n = (int)cdiv(g.N, BN);
     dim3 grid(g.tiles_m * g.tiles_n, batch);
     dim3 block((BM / WM) * (BN / WN) * 64);
+    if (KONLY || kvec_ok(g, amc, bnc)) {
+        hipLaunchKernelGGL((gemm_bf16k_kernel<BM, BN, BK, WM, WN>), grid, block, 0, s, g);
+        return check_launch();
+    }
+    if constexpr (!KONLY) {
     if (amc && bnc) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, true>), grid, block, 0, s, g);
     else if (amc) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, false>), grid, block, 0, s, g);
     else if (bnc) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, true>), grid, block, 0, s, g);
     else hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, false>), grid, block, 0, s, g);
+    }
     return check_launch();
+}
+
+// Tile configuration table.  config -1 selects by shape (the default
+// heuristic).  Every fp32 configuration uses the same MFMA instruction and
+// walks K in the same order, so the choice never changes the result bits.
+static int gemm_dispatch(int dtype, int compute, int config, GemmArgs g, int batch, bool amc, bool bnc,
+                         hipStream_t s) {
+    const long long work = (long long)g.M * g.N;
+    if (dtype == TRG_F32 && compute == TRG_F32) {
+        if (config < 0) {
+            if (g.N <= 64 && g.M >= 4096) config = 3;
+            else if (g.M <= 64 && g.N >= 4096) config = 4;
+            else if (work >= 256LL * 128 * 128) config = 0;
+            else config = 2;
+        }
+        switch (config) {
+            case 0: return launch_t<float, 128, 128, 32, 64, 64>(g, batch, amc, bnc, s);
+            case 1: return launch_t<float, 128, 128, 16, 64, 64>(g, batch, amc, bnc, s);
+            case 2: return launch_t<float, 64, 64, 16, 32, 32>(g, batch, amc, bnc, s);
+            case 3: return launch_t<float, 256, 64, 16, 64, 64>(g, batch, amc, bnc, s);
+            case 4: return launch_t<float, 64, 256, 16, 64, 64>(g, batch, amc, bnc, s);
+            case 5: return launch_t<float, 256, 128, 32, 64, 64>(g, batch, amc, bnc, s);
+            case 6: return launch_t<float, 128, 256, 32, 64, 64>(g, batch, amc, bnc, s);
+            case 7: return launch_t<float, 256, 128, 16, 128, 64>(g, batch, amc, bnc, s);
+            default: return TRG_BAD_ARG;
+        }
+    }
+    if (dtype == TRG_F32 && compute == TRG_BF16) {
+        if (config < 0) config = work >= 256LL * 128 * 128 ? 0 : 1;
+        switch (config) {
+            case 0: return launch_bf16<128, 128, 32, 64, 64>(g, batch, amc, bnc, s);
+            case 1: return launch_bf16<64, 64, 32, 32, 32>(g, batch, amc, bnc, s);
+            case 2: return launch_bf16<256, 128, 32, 64, 64>(g, batch, amc, bnc, s);
+            case 3: return launch_bf16<256, 128, 32, 128, 64>(g, batch, amc, bnc, s);
+            case 4: return launch_bf16<128, 256, 32, 64, 128>(g, batch, amc, bnc, s);
+            case 5: return launch_bf16<128, 128, 64, 64, 64>(g, batch, amc, bnc, s);
+            case 6: return launch_bf16<256, 128, 64, 128, 64, true>(g, batch, amc, bnc, s);
+            case 7: return launch_bf16<256, 256, 32, 128, 64, true>(g, batch, amc, bnc, s);
+            case 8: return launch_bf16<256, 256, 16, 128, 64>(g, batch, amc, bnc, s);
+            case 9: return launch_bf16<128, 256, 32, 64, 64>(g, batch, amc, bnc, s);
+            default: return TRG_BAD_ARG;
+        }
+    }
+    if (dtype == TRG_F64 && compute == TRG_F64) {
+        if (config < 0) config = work >= 256LL * 128 * 128 ? 0 : 1;
+        switch (config) {
+            case 0: return launch_t<double, 128, 128, 16, 64, 64>(g, batch, amc, bnc, s);
+            case 1: return launch_t<double, 64, 64, 16, 32, 32>(g, batch, amc, bnc, s);
+            case 2: return launch_t<double, 128, 64, 16, 64, 32>(g, batch, amc, bnc, s);
+            case 3: return launch_t<double, 64, 128, 16, 32, 64>(g, batch, amc, bnc, s);
+            default: return TRG_BAD_ARG;
+        }
+    }
+    return TRG_UNSUPPORTED;
 }
 
 }  // namespace trg
 
 using namespace trg;
+
+extern "C" int trg_gemm_config_count(int dtype, int compute);
 
 extern "C" int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, double alpha,
                         const void *A, long long sAb, long long rsA, long long csA,
@@ -444,20 +653,43 @@ extern "C" int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, 
         // C = beta * C: a GEMM with an empty sum.  Use the kernel with K = 0 (loop skipped).
     }
     GemmArgs g{M, N, K, A, sAb, rsA, csA, B, sBb, rsB, csB, C, sCb, rsC, csC, alpha, beta, 0, 0};
-    const long long work = (long long)M * N;
-    if (dtype == TRG_F32 && compute == TRG_F32) {
-        if (N <= 64 && M >= 4096) return launch_t<float, 256, 64, 16, 64, 64>(g, batch, amc, bnc, s);
-        if (M <= 64 && N >= 4096) return launch_t<float, 64, 256, 16, 64, 64>(g, batch, amc, bnc, s);
-        if (work >= 256LL * 128 * 128) return launch_t<float, 128, 128, 32, 64, 64>(g, batch, amc, bnc, s);
-        return launch_t<float, 64, 64, 16, 32, 32>(g, batch, amc, bnc, s);
-    }
-    if (dtype == TRG_F32 && compute == TRG_BF16) {
-        if (work >= 256LL * 128 * 128) return launch_bf16<128, 128, 32, 64, 64>(g, batch, amc, bnc, s);
-        return launch_bf16<64, 64, 32, 32, 32>(g, batch, amc, bnc, s);
-    }
-    if (dtype == TRG_F64 && compute == TRG_F64) {
-        if (work >= 256LL * 128 * 128) return launch_t<double, 128, 128, 16, 64, 64>(g, batch, amc, bnc, s);
-        return launch_t<double, 64, 64, 16, 32, 32>(g, batch, amc, bnc, s);
-    }
-    return TRG_UNSUPPORTED;
+    return gemm_dispatch(dtype, compute, -1, g, batch, amc, bnc, s);
+}
+
+extern "C" int trg_gemm_config_count(int dtype, int compute) {
+    if (dtype == TRG_F32 && compute == TRG_F32) return 8;
+    if (dtype == TRG_F32 && compute == TRG_BF16) return 10;
+    if (dtype == TRG_F64 && compute == TRG_F64) return 4;
+    return 0;
+}
+
+extern "C" const char *trg_gemm_config_name(int dtype, int compute, int config) {
+    static const char *f32[] = {"128x128x32 w64x64", "128x128x16 w64x64", "64x64x16 w32x32", "256x64x16 w64x64",
+                                "64x256x16 w64x64", "256x128x32 w64x64", "128x256x32 w64x64", "256x128x16 w128x64"};
+    static const char *bf[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "256x128x32 w128x64",
+                               "128x256x32 w64x128", "128x128x64 w64x64", "256x128x64 w128x64",
+                               "256x256x32 w128x64", "256x256x16 w128x64", "128x256x32 w64x64"};
+    static const char *f64[] = {"128x128x16 w64x64", "64x64x16 w32x32", "128x64x16 w64x32", "64x128x16 w32x64"};
+    if (config < 0 || config >= trg_gemm_config_count(dtype, compute)) return "";
+    if (dtype == TRG_F32 && compute == TRG_F32) return f32[config];
+    if (dtype == TRG_F32 && compute == TRG_BF16) return bf[config];
+    return f64[config];
+}
+
+extern "C" int trg_gemm_ex(int config, int dtype, int compute, int batch, int M, int N, int K, double alpha,
+                           const void *A, long long sAb, long long rsA, long long csA,
+                           const void *B, long long sBb, long long rsB, long long csB,
+                           double beta, void *C, long long sCb, long long rsC, long long csC,
+                           void *stream) {
+    if (batch < 0 || M < 0 || N < 0 || K < 0) return TRG_BAD_ARG;
+    if (batch == 0 || M == 0 || N == 0) return TRG_OK;
+    if (!A || !B || !C) return TRG_BAD_ARG;
+    if (batch > 65535) return TRG_SHAPE;
+    if (config < -1 || config >= trg_gemm_config_count(dtype, compute)) return TRG_BAD_ARG;
+    const bool amc = (rsA == 1);
+    const bool bnc = (csB == 1);
+    if (!amc && csA != 1) return TRG_BAD_ARG;
+    if (!bnc && rsB != 1) return TRG_BAD_ARG;
+    GemmArgs g{M, N, K, A, sAb, rsA, csA, B, sBb, rsB, csB, C, sCb, rsC, csC, alpha, beta, 0, 0};
+    return gemm_dispatch(dtype, compute, config, g, batch, amc, bnc, as_stream(stream));
 }

@@ -57,22 +57,30 @@ class Policy:
     weight 2^-n (lnf[n] = ln c_n / 2^n and T_n only feeds later terms), so
     early steps run in float64 and later ones in lower precision."""
 
-    def __init__(self, fp64_steps=10**9, low=FP32, bf16_env_from=None):
+    def __init__(self, fp64_steps=10**9, low=FP32, bf16_env_from=None, bf16_merge_from=None):
         self.fp64_steps = fp64_steps
         self.low = low
         self.bf16_env_from = bf16_env_from
+        self.bf16_merge_from = bf16_merge_from
 
     def __call__(self, step):
         if step <= self.fp64_steps:
             return FP64
+        env, merge = self.low.env, self.low.merge
         if self.bf16_env_from is not None and step > self.bf16_env_from:
-            return Precision(self.low.store, "bf16", self.low.merge)
-        return self.low
+            env = "bf16"
+        if self.bf16_merge_from is not None and step > self.bf16_merge_from:
+            merge = "bf16"
+        if (env, merge) == (self.low.env, self.low.merge):
+            return self.low
+        return Precision(self.low.store, env, merge)
 
     def describe(self):
         s = f"fp64 for steps <= {self.fp64_steps}, then {self.low.store} env={self.low.env} merge={self.low.merge}"
         if self.bf16_env_from is not None:
             s += f", bf16 env after step {self.bf16_env_from}"
+        if self.bf16_merge_from is not None:
+            s += f", bf16 merge after step {self.bf16_merge_from}"
         return s
 
 
@@ -176,7 +184,13 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
         for i, b0 in enumerate(range(0, B, nsub)):
             ops.copy_into(res[b0:b0 + outs[i].shape[0]], outs[i])
         return res
-    Tp = permute(T2, (0, 3, 1, 2, 4)).view(B, Dy * Dx, Dx * Dy)          # [(e,b),(d,l)]
+    # fp32 MFMA wants K-outer operands, bf16 MFMA K-contiguous ones (8
+    # consecutive k per lane); the permutations below produce either layout.
+    kinner = compute == "bf16"
+    if kinner:
+        Tp = permute(T2, (0, 2, 4, 3, 1)).view(B, Dx * Dy, Dy * Dx).transpose(1, 2)   # [(e,b),(d,l)]
+    else:
+        Tp = permute(T2, (0, 3, 1, 2, 4)).view(B, Dy * Dx, Dx * Dy)      # [(e,b),(d,l)]
     T1m = T1.reshape(B, Dx, Dx * Dy * Dy)                                 # [a,(c,k,e)]
     Uam = Ua.reshape(B, Dx, Dx * chi_a)                                   # [a,(b,i)]
     Ubm = Ub.reshape(B, Dx * Dx, chi_b)                                   # [(c,d),j]
@@ -187,9 +201,13 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
         nc = c1 - c0
         A = T1m[:, :, c0 * Dy * Dy:c1 * Dy * Dy].transpose(1, 2)          # (B, (c,k,e), a)
         Z1 = gemm(A, Uam, compute=compute)                                 # [(c,k,e),(b,i)]
-        Z1p = permute(Z1.view(B, nc, Dy, Dy, Dx, chi_a), (0, 3, 4, 1, 2, 5))   # [e,b,c,k,i]
-        Z2 = gemm(Z1p.view(B, Dy * Dx, nc * Dy * chi_a).transpose(1, 2), Tp, compute=compute,
-                  tag="merge2d_z2")                                        # [(c,k,i),(d,l)]
+        if kinner:
+            Z1p = permute(Z1.view(B, nc, Dy, Dy, Dx, chi_a), (0, 1, 2, 5, 3, 4))   # [c,k,i,e,b]
+            Z2a = Z1p.view(B, nc * Dy * chi_a, Dy * Dx)
+        else:
+            Z1p = permute(Z1.view(B, nc, Dy, Dy, Dx, chi_a), (0, 3, 4, 1, 2, 5))   # [e,b,c,k,i]
+            Z2a = Z1p.view(B, Dy * Dx, nc * Dy * chi_a).transpose(1, 2)
+        Z2 = gemm(Z2a, Tp, compute=compute, tag="merge2d_z2")             # [(c,k,i),(d,l)]
         Z2p = permute(Z2.view(B, nc, Dy, chi_a, Dx, Dy), (0, 2, 3, 5, 1, 4))   # [k,i,l,c,d]
         gemm(Z2p.view(B, Dy * chi_a * Dy, nc * Dx), Ubm[:, c0 * Dx:c1 * Dx, :], outm,
              beta=0.0 if c0 == 0 else 1.0, compute=compute)

@@ -5,7 +5,7 @@ arithmetic operation below is a libtrg kernel.
 """
 import torch
 
-from . import _lib
+from . import _lib, autotune
 from ._lib import BF16, F32, F64, call, ll_array
 
 _DT = {torch.float32: F32, torch.float64: F64}
@@ -106,6 +106,7 @@ class GemmRecorder:
     def __init__(self, tag):
         self.tag = tag
         self.events = []   # (start, end, flops)
+        self.configs = set()   # (dtype, compute, tile config, M, N, K, batch) launched
 
     def __enter__(self):
         _RECORDERS.append(self)
@@ -181,16 +182,25 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None):
         cc = compute
     if dt == F64:
         cc = F64
+    opnds = (ptr(A), sA, rsA, csA, ptr(B), sB, rsB, csB)
+    config = -1
+    if autotune.active():
+        def launch(cfg, out):
+            call("trg_gemm_ex", cfg, dt, cc, batch, M, N, K, float(alpha), *opnds,
+                 0.0, ptr(out), M * N if batch > 1 else 0, N, 1, stream_ptr())
+        own = beta == 0.0 and C.is_contiguous() and C.shape == (batch, M, N)
+        config = autotune.choose(dt, cc, batch, M, N, K, opnds, launch, C if own else None)
     recs = [r for r in _RECORDERS if tag is not None and r.tag == tag]
     if recs:
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record()
-    call("trg_gemm", dt, cc, batch, M, N, K, float(alpha), ptr(A), sA, rsA, csA, ptr(B), sB, rsB, csB,
+    call("trg_gemm_ex", config, dt, cc, batch, M, N, K, float(alpha), *opnds,
          float(beta), ptr(C), C.stride(0) if batch > 1 else 0, C.stride(1), C.stride(2), stream_ptr())
     if recs:
         ev1.record()
         for r in recs:
             r.events.append((ev0, ev1, 2.0 * M * N * K * batch))
+            r.configs.add((dt, cc, config, M, N, K, batch))
     return C if batched else C[0]
 
 
