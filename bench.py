@@ -1,23 +1,33 @@
 """Benchmark: batched mixed-precision HOTRG 2D on one MI355X.
 
-Workload (one line of the JSON contract): 2D Ising HOTRG, chi = 64, 32
-couplings beta_i = 0.40 + 0.08 i / 31 batched into every launch, RG steps
-alternating Y/X.  The instances are grown untimed to the steady shape
-(bonds chi x chi); W warm-up steps and then K timed steps follow, each one a
-coarse-graining of all 32 instances along one axis in float32 on MFMA with
-float64 isometries (eigensolver), traces and log-norms.
+Workload (the JSON line's config): 2D Ising HOTRG, chi = 64, 40 RG steps (Y first, then
+alternating; 20 per axis), 32 couplings beta_i = 0.40 + 0.08 i / 31 batched into every launch,
+starting from the seeded initial tensor T = sum_s W^4 (D0 = 2).  A "step" of this bench is one
+RG step of all 32 instances along one axis.  The timed region is the whole solve: steps 1..K of
+the schedule (K = 40 by default: bond growth, float64 steps, float32 steps, bf16-merge steps)
+with the inputs resident in HBM, then ln Z/V, closed by a device synchronisation.
 value = 32 * K / (timed seconds)  [RG steps/s, whole GPU].
 
-Also reported on the same line:
-  * roofline of the dominant kernel (the chi^7 GEMM of the merge), its
-    launches timed live with HIP events on the launching stream during the
-    timed region; algorithmic flops 2 M N K per launch;
-  * lnZ_err_vs_fp64: |ln Z/V (mixed) - ln Z/V (float64 path)| of full 40-step
-    solves for two couplings (untimed);
-  * plain_fp64_steps_per_s: one instance, float64 throughout, host sync per
-    step (the per-instance float64 GPU path);
-  * cpu_baseline: the NumPy float64 oracle timed on the host for one
-    steady-state chi = 64 RG step of one instance.
+Precision schedule (hotrg.Policy): steps 1..F in float64 (F = --fp64-steps), then float32 on
+MFMA with float64 isometry eigensolvers, traces and log-norms; after step --bf16-merge-from the
+chi^7 merge GEMM runs on bf16 MFMA with float32 accumulation.  An error made at step n enters
+ln Z/V with weight 2^-n.
+
+Warm-up (untimed): the same schedule with every precision tier after step 6 cut to its first
+W steps, which loads every kernel and lets the GEMM autotuner time each tile configuration on
+every problem shape the timed solve meets.
+
+Also on the JSON line:
+  * roofline: the kernel with the most time in the timed solve (the chi^7 merge GEMM), its
+    launches timed live with HIP events on the launching stream; algorithmic flops 2 M N K per
+    launch; the merge GEMMs of the other precisions are listed beside it;
+  * lnZ_err_vs_fp64: max |ln Z/V (timed solve) - ln Z/V (float64 solve)| over a subset of the
+    couplings (all of them with --validate-all);
+  * plain_fp64: one instance, float64 throughout, a host sync per step, the same solve (the
+    per-instance float64 GPU path); speedup_vs_plain_fp64 = value / its steps/s;
+  * serving: a Poisson stream of mixed solves through serve.Server (p50/p99 latency);
+  * cpu_baseline: the NumPy float64 oracle timed on the host for one steady-state chi = 64 RG
+    step of one instance.
 """
 import argparse
 import json
@@ -28,25 +38,216 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3     # MI355X dense fp32 matrix peak
-HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"f32": 157.3, "bf16": 2500.0, "f64": 78.6}   # MI355X dense MFMA peaks
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--chi", type=int, default=64)
-    ap.add_argument("--nbeta", type=int, default=32)
-    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--workload", default="hotrg2d", choices=["hotrg2d", "hotrg3d", "atrg3d"],
+                    help="hotrg2d: the headline line; hotrg3d / atrg3d: the batched 3D Ising HOTRG and the "
+                         "3D q=3 Potts ATRG configurations (secondary lines)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed RG steps of the schedule (hotrg2d 40 = whole solve; hotrg3d 6; atrg3d 30)")
+    ap.add_argument("--warmup", type=int, default=2, help="warm-up steps per precision tier")
+    ap.add_argument("--chi", type=int, default=None, help="hotrg2d 64, hotrg3d 16, atrg3d 64")
+    ap.add_argument("--nbeta", type=int, default=None, help="hotrg2d 32, hotrg3d 32 (of the 128-value draw)")
+    ap.add_argument("--fp64-steps", type=int, default=None, help="hotrg2d 6, hotrg3d 3")
+    ap.add_argument("--atrg-q", type=int, default=2)
+    ap.add_argument("--atrg-prec", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--bf16-merge-from", type=int, default=20, help="bf16 merge after this step (-1: never)")
+    ap.add_argument("--validate-betas", type=int, default=4)
+    ap.add_argument("--validate-all", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-n", type=int, default=16)
+    ap.add_argument("--serve-n", type=int, default=24)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run one float32 and one bf16-merge steady step plus a 1 GiB copy of known size "
+                         "(for hardware-counter runs), then exit")
     return ap.parse_args()
+
+
+def _line(metric_cfg, value, K, warmup, dt, dtype, roof, extra):
+    out = {"metric": "RG steps/sec and p50/p99 solve latency at |dlnZ/V|<=1e-8 vs FP64 plain path",
+           "value": value, "unit": "RG steps/s", "n_gpus": 1, "steps": K, "warmup": warmup,
+           "ms_per_step": dt / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": dtype, "data": "synthetic", "config": metric_cfg}
+    if roof is not None:
+        out["roofline"] = roof
+    out.update(extra)
+    return out
+
+
+def _roof(rec, names, peak):
+    import torch  # noqa: F401  (events need the runtime)
+    evs = [(e0.elapsed_time(e1), fl) for e0, e1, fl in rec.events]
+    if not evs:
+        return None
+    ms = sum(e[0] for e in evs)
+    fl = sum(e[1] for e in evs)
+    ach = fl / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+            "traffic": None, "kernel": names, "launches": len(evs), "avg_ms": ms / len(evs),
+            "flops_per_launch": fl / len(evs)}
+
+
+def bench_hotrg3d(args):
+    """Batched 3D Ising HOTRG: beta drawn from numpy.random.default_rng(0).uniform(0.20, 0.24, 128)
+    (the first --nbeta of them), chi = 16, steps Z, Y, X (trg/HOTRG_3d.py:259-316) from the initial
+    tensor; float64 for the first --fp64-steps steps, then float32 MFMA with float64 eigensolvers."""
+    import numpy as np
+    import torch
+
+    from trg_backup_amd import _lib, autotune, hotrg, models, ops
+    _lib.load()
+    dev = torch.device("cuda:0")
+    chi, nb, K = args.chi, args.nbeta, args.steps
+    betas = np.random.default_rng(0).uniform(0.20, 0.24, 128)[:nb].tolist()
+    policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32)
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    T0 = torch.stack([models.initial_tensor("ising3d", b, dev) for b in betas])
+    sched = [(n, "ZYX"[(n - 1) % 3]) for n in range(1, K + 1)]
+
+    def run(steps, pol, T0_):
+        lnf = ops.fill_(torch.empty((T0_.shape[0], K + 1), dtype=torch.float64, device=dev), 0.0)
+        T = ops.contiguous(T0_, pol(0).store)
+        hotrg.normalize_trace(T, lnf, 0)
+        for n, d in steps:
+            T = hotrg.step3d(T, chi, d, pol(n))
+            hotrg.normalize_trace(T, lnf, n)
+        return T, lnf
+
+    autotune.enable(not args.no_tune)
+    t0 = time.perf_counter()
+    run(sched[:max(args.fp64_steps, 3) + min(3, args.warmup + 1)], policy, T0)
+    torch.cuda.synchronize()
+    autotune.enable(False)
+    log(f"warm-up in {time.perf_counter() - t0:.1f} s")
+    torch.cuda.empty_cache()
+    rec = ops.GemmRecorder("merge3d_x3")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with rec:
+        T, lnf = run(sched, policy, T0)
+        z = hotrg.lnz3d(T, lnf, K)
+        torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    zt = z.cpu().numpy()
+    log(f"timed {K} steps x {nb} instances in {dt:.2f} s")
+    del T
+    torch.cuda.empty_cache()
+    extra = {"lnZ_per_site": {"betas": betas[:3], "values": zt[:3].tolist()}}
+    if not args.no_validate:
+        sel = [0, nb - 1] if nb > 1 else [0]
+        Td, ld = run(sched, hotrg.ALL_FP64, ops.contiguous(T0[sel]))
+        zd = hotrg.lnz3d(Td, ld, K).cpu().numpy()
+        extra["lnZ_err_vs_fp64"] = float(max(abs(zt[i] - zd[j]) for j, i in enumerate(sel)))
+        del Td
+    cfg = {"workload": f"3D Ising HOTRG chi={chi}, {K} RG steps (Z,Y,X) from the initial tensor, {nb} beta "
+                       "from default_rng(0).uniform(0.20,0.24) batched",
+           "chi": chi, "nbeta": nb, "rg_steps": K, "policy": policy.describe(), "parallelism": "single GPU, batched"}
+    roof = _roof(rec, "merge chi^11 GEMM (T.U1.U2 x T, f32 MFMA)", PEAK_TFLOPS["f32"])
+    print(json.dumps(_line(cfg, nb * K / dt, K, args.warmup, dt, "f32", roof, extra)), flush=True)
+    return 0
+
+
+def bench_atrg3d(args):
+    """3D q=3 Potts ATRG, beta = 0.55, chi = 64, steps T, X, Y (trg/ATRG_3d_new.py:974-1039) from the
+    eigen-factorised initial tensor; leg-swap rSVD with k = chi, p = chi, --atrg-q power iterations,
+    Philox sketch from seed 1; contractions in --atrg-prec, decompositions float64."""
+    import math
+
+    import torch
+
+    from trg_backup_amd import _lib, atrg, autotune, models, ops
+    _lib.load()
+    chi, K = args.chi, args.steps
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    saved = dict(atrg.RSVD)
+
+    def solve(nsteps, prec, q):
+        atrg.set_rsvd_params(k_factor=1, p=chi, q=q, seed=1)
+        U, s, VH = models.atrg_initial("potts3d", 0.55)
+        s, c = atrg.normalize(U, s, VH)
+        consts = [c]
+        for n in range(1, nsteps + 1):
+            U, s, VH = atrg.atrg_pure_tensor(U, s, VH, chi, "TXY"[(n - 1) % 3], prec)
+            s, c = atrg.normalize(U, s, VH)
+            consts.append(c)
+        tr = atrg.trace(U, s, VH)
+        lg = sum(math.log(abs(float(c[0].item()))) / 2 ** k for k, c in enumerate(consts))
+        return lg + math.log(abs(float(tr[0].item()))) / 2 ** nsteps
+
+    try:
+        autotune.enable(not args.no_tune)
+        t0 = time.perf_counter()
+        solve(min(K, 3 + args.warmup), args.atrg_prec, args.atrg_q)
+        torch.cuda.synchronize()
+        autotune.enable(False)
+        log(f"warm-up in {time.perf_counter() - t0:.1f} s")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        z = solve(K, args.atrg_prec, args.atrg_q)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log(f"timed {K} steps in {dt:.2f} s, ln Z/V = {z!r}")
+        extra = {"lnZ_per_site": {"beta": 0.55, "value": z}}
+        if not args.no_validate:
+            t1 = time.perf_counter()
+            zd = solve(K, "f64", args.atrg_q)
+            extra["lnZ_err_vs_fp64"] = abs(z - zd)
+            extra["fp64_same_q"] = {"value": zd, "steps_per_s": K / (time.perf_counter() - t1)}
+    finally:
+        atrg.RSVD.update(saved)
+    cfg = {"workload": f"3D q=3 Potts ATRG chi={chi}, beta=0.55, {K} RG steps (T,X,Y), rSVD k=chi p=chi "
+                       f"q={args.atrg_q} seed 1", "chi": chi, "rg_steps": K, "precision": args.atrg_prec,
+           "parallelism": "single GPU, one instance"}
+    print(json.dumps(_line(cfg, K / dt, K, args.warmup, dt, args.atrg_prec, None, extra)), flush=True)
+    return 0
+
+
+def traffic_probe(args):
+    """One steady float32 step and one bf16-merge step of the bench batch, then a plain
+    1 GiB float32 copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    import torch
+
+    from trg_backup_amd import _lib, autotune, hotrg, models, ops
+    _lib.load()
+    dev = torch.device("cuda:0")
+    chi, nb = args.chi, args.nbeta
+    betas = [0.40 + 0.08 * i / (nb - 1) for i in range(nb)] if nb > 1 else [0.44]
+    T = ops.contiguous(torch.stack([models.initial_tensor("ising2d", b, dev) for b in betas]))
+    lnf = ops.fill_(torch.empty((nb, 16), dtype=torch.float64, device=dev), 0.0)
+    n, d = 0, "Y"
+    while T.shape[1] < chi or T.shape[3] < chi:
+        n += 1
+        T = hotrg.step2d(T, chi, d, hotrg.FP64)
+        hotrg.normalize_maxabs(T, lnf, n)
+        d = "X" if d == "Y" else "Y"
+    T = ops.cast(T, torch.float32)
+    autotune.enable(not args.no_tune)
+    for prec in (hotrg.FP32, hotrg.Precision(torch.float32, "f32", "bf16")):
+        # tune first (untimed), then the counted step
+        hotrg.step2d(T, chi, d, prec)
+        torch.cuda.synchronize()
+        print(f"[probe] step {d} {prec} tiles {autotune.table()}", flush=True)
+        T = hotrg.step2d(T, chi, d, prec)
+        torch.cuda.synchronize()
+    autotune.enable(False)
+    src = ops.fill_(torch.empty(1 << 28, dtype=torch.float32, device=dev), 1.0)
+    dst = torch.empty_like(src)
+    ops.copy_into(dst, src)
+    torch.cuda.synchronize()
+    print("[probe] copy of 1 GiB float32 done", flush=True)
+    return 0
+
+
+def schedule(nsteps):
+    """(n, direction) as trg/HOTRG_2d.py:195-214 runs them: Y first, then alternating."""
+    return [(n, "Y" if n % 2 == 1 else "X") for n in range(1, nsteps + 1)]
 
 
 def main():
@@ -55,6 +256,17 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(independent couplings shard trivially across GPUs, see DESIGN.md)"}))
         return 2
+    wl = args.workload
+    args.chi = args.chi or {"hotrg2d": 64, "hotrg3d": 16, "atrg3d": 64}[wl]
+    args.nbeta = args.nbeta or (1 if wl == "atrg3d" else 32)
+    args.steps = args.steps or {"hotrg2d": 40, "hotrg3d": 6, "atrg3d": 30}[wl]
+    args.fp64_steps = args.fp64_steps if args.fp64_steps is not None else {"hotrg2d": 6, "hotrg3d": 3}.get(wl, 0)
+    if args.traffic_probe:
+        return traffic_probe(args)
+    if wl == "hotrg3d":
+        return bench_hotrg3d(args)
+    if wl == "atrg3d":
+        return bench_atrg3d(args)
     import numpy as np
     import torch
 
@@ -62,120 +274,135 @@ def main():
 
     _lib.load()
     dev = torch.device("cuda:0")
-    chi, nb = args.chi, args.nbeta
+    chi, nb, K = args.chi, args.nbeta, args.steps
     betas = [0.40 + 0.08 * i / (nb - 1) for i in range(nb)] if nb > 1 else [0.44]
-    policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32)
+    policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32,
+                          bf16_merge_from=None if args.bf16_merge_from < 0 else args.bf16_merge_from)
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    T0 = torch.stack([models.initial_tensor("ising2d", b, dev) for b in betas])
 
-    # ---- growth to the steady shape (untimed) ----
-    T = torch.stack([models.initial_tensor("ising2d", b, dev) for b in betas])
-    nsteps_total = 6 + args.warmup + args.steps
-    lnf = ops.fill_(torch.empty((nb, nsteps_total + 1), dtype=torch.float64, device=dev), 0.0)
-    T = ops.contiguous(T)
-    hotrg.normalize_maxabs(T, lnf, 0)
-    step = 0
-    direction = "Y"
-    while T.shape[1] < chi or T.shape[3] < chi:
-        step += 1
-        T = hotrg.step2d(T, chi, direction, hotrg.FP64)
-        hotrg.normalize_maxabs(T, lnf, step)
-        direction = "X" if direction == "Y" else "Y"
-    T = ops.cast(T, torch.float32)
-    torch.cuda.synchronize()
-    log(f"steady shape {tuple(T.shape)} after {step} growth steps")
-
-    def rg_step(T, direction, st):
-        T = hotrg.step2d(T, chi, direction, hotrg.FP32)
-        hotrg.normalize_maxabs(T, lnf, st)
+    def run(steps, lnf):
+        T = ops.contiguous(T0)
+        hotrg.normalize_maxabs(T, lnf, 0)
+        for n, d in steps:
+            T = hotrg.step2d(T, chi, d, policy(n))
+            hotrg.normalize_maxabs(T, lnf, n)
         return T
 
-    # warm-up steps also pick the GEMM tile configurations (tuning is untimed)
+    # ---- warm-up: every precision tier after step 6 cut to its first W steps (untimed) ----
+    warm, seen = [], {}
+    for n, d in schedule(K):
+        p = policy(n)
+        key = (p.store, p.env, p.merge)
+        if n <= 6 or seen.get(key, 0) < args.warmup:
+            warm.append((n, d))
+            if n > 6:
+                seen[key] = seen.get(key, 0) + 1
     autotune.enable(not args.no_tune)
-    for _ in range(args.warmup):
-        step += 1
-        T = rg_step(T, direction, step)
-        direction = "X" if direction == "Y" else "Y"
+    t0 = time.perf_counter()
+    lw = ops.fill_(torch.empty((nb, K + 1), dtype=torch.float64, device=dev), 0.0)
+    run(warm, lw)
     torch.cuda.synchronize()
     autotune.enable(False)
+    log(f"warm-up {len(warm)} steps in {time.perf_counter() - t0:.1f} s; "
+        f"{len(autotune.table())} GEMM problems tuned")
+    del lw
+    torch.cuda.empty_cache()
 
+    # ---- timed: the whole solve ----
+    lnf = ops.fill_(torch.empty((nb, K + 1), dtype=torch.float64, device=dev), 0.0)
     rec = ops.GemmRecorder("merge2d_z2")
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     with rec:
-        for _ in range(args.steps):
-            step += 1
-            T = rg_step(T, direction, step)
-            direction = "X" if direction == "Y" else "Y"
+        T = run(schedule(K), lnf)
+        z = hotrg.lnz2d(T, lnf, K)
         torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    value = nb * args.steps / dt
-    g = rec.summary()
-    achieved = g["avg_flops"] / (g["avg_ms"] * 1e-3) / 1e12
-    log(f"timed {args.steps} steps x {nb} instances in {dt:.3f} s; z2 gemm {g}")
+    value = nb * K / dt
+    zt = z.cpu().numpy()
+    log(f"timed {K} steps x {nb} instances in {dt:.2f} s -> {value:.2f} RG steps/s")
+    del T
+    torch.cuda.empty_cache()
+
+    # merge GEMM launches grouped by (storage, compute)
+    names = {(_lib.F32, _lib.F32): "f32", (_lib.F32, _lib.BF16): "bf16", (_lib.F64, _lib.F64): "f64"}
+    groups = {}
+    for (e0, e1, fl), cfg in zip(rec.events, rec.event_configs):
+        groups.setdefault(names[cfg[:2]], []).append((e0.elapsed_time(e1), fl, cfg))
+    kern = {}
+    for nm, evs in groups.items():
+        ms = sum(e[0] for e in evs)
+        fl = sum(e[1] for e in evs)
+        ach = fl / (ms * 1e-3) / 1e12
+        kern[nm] = {"launches": len(evs), "avg_ms": ms / len(evs), "total_ms": ms,
+                    "flops_per_launch": fl / len(evs), "achieved_tflops": ach,
+                    "peak_tflops": PEAK_TFLOPS[nm], "frac": ach / PEAK_TFLOPS[nm],
+                    "tiles": sorted({autotune.config_name(c[0], c[1], c[2]) for _, _, c in evs})}
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    d = kern[dom]
 
     traffic = None
     tp = os.path.join(ROOT, "profiles", "gemm_merge2d_z2_traffic.json")
     if os.path.exists(tp):
         with open(tp) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            traffic = json.load(f).get("bytes_per_launch", {}).get(dom)
 
     out = {
         "metric": "RG steps/sec and p50/p99 solve latency at |dlnZ/V|<=1e-8 vs FP64 plain path",
-        "value": value, "unit": "RG steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "value": value, "unit": "RG steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": dt / K * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"2D Ising HOTRG chi={chi}, {nb} beta in [0.40,0.48] batched, "
-                               "steady-state RG steps (Y/X alternating)",
-                   "chi": chi, "nbeta": nb, "policy": policy.describe(), "parallelism": "single GPU, batched"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "trg::gemm_kernel<float,...> tiles "
-                               + ", ".join(sorted({autotune.config_name(a, b, c) for a, b, c, *_ in rec.configs}))
-                               + " (merge chi^7 GEMM, M=nc*chi^2 N=chi^2 K=chi^2 batch=nbeta)",
-                     "launches": g["launches"], "avg_ms": g["avg_ms"], "flops_per_launch": g["avg_flops"]},
+        "config": {"workload": f"2D Ising HOTRG chi={chi}, {K} RG steps from the initial tensor "
+                               f"(Y first, alternating), {nb} beta in [0.40,0.48] batched",
+                   "chi": chi, "nbeta": nb, "rg_steps": K, "policy": policy.describe(),
+                   "parallelism": "single GPU, batched"},
+        "roofline": {"bound": "mfma", "achieved": d["achieved_tflops"], "peak": d["peak_tflops"],
+                     "unit": "TFLOP/s", "frac": d["frac"], "traffic": traffic,
+                     "kernel": f"merge chi^7 GEMM ({dom} MFMA, tiles {', '.join(d['tiles'])}; "
+                               "M=nc*chi^2 N=chi^2 K=chi^2, batch=nbeta)",
+                     "launches": d["launches"], "avg_ms": d["avg_ms"], "flops_per_launch": d["flops_per_launch"],
+                     "merge_gemm_by_precision": kern},
     }
-    del T
-    torch.cuda.empty_cache()
+    out["lnZ_per_site"] = {"betas": [betas[0], betas[nb // 2], betas[-1]],
+                           "values": [float(zt[0]), float(zt[nb // 2]), float(zt[-1])]}
 
-    # ---- per-instance float64 path with a host sync per step ----
+    # ---- accuracy against the float64 path (untimed) ----
+    if not args.no_validate:
+        idx = list(range(nb)) if args.validate_all else sorted(
+            set(np.linspace(0, nb - 1, max(2, args.validate_betas)).round().astype(int).tolist()) | {nb // 2})
+        errs = []
+        for c0 in range(0, len(idx), 8):
+            sel = idx[c0:c0 + 8]
+            Td, ld = hotrg.renorm2d(ops.contiguous(T0[sel]), chi, K // 2, K - K // 2, hotrg.ALL_FP64)
+            zd = hotrg.lnz2d(Td, ld, K).cpu().numpy()
+            errs += [abs(float(zt[i]) - float(zd[j])) for j, i in enumerate(sel)]
+            del Td
+            torch.cuda.empty_cache()
+            log(f"validated betas {[round(betas[i], 4) for i in sel]}: max err so far {max(errs):.3e}")
+        out["lnZ_err_vs_fp64"] = max(errs)
+        out["lnZ_check"] = {"betas": [betas[i] for i in idx], "abs_err": errs}
+
+    # ---- plain per-instance float64 path: the same solve, one instance, host sync per step ----
     if not args.no_plain:
-        T1 = models.initial_tensor("ising2d", 0.44, dev)[None].contiguous()
-        l1 = ops.fill_(torch.empty((1, 64), dtype=torch.float64, device=dev), 0.0)
-        s, d = 0, "Y"
-        while T1.shape[1] < chi or T1.shape[3] < chi:
-            s += 1
-            T1 = hotrg.step2d(T1, chi, d, hotrg.FP64)
-            hotrg.normalize_maxabs(T1, l1, s)
-            d = "X" if d == "Y" else "Y"
+        T1 = ops.contiguous(T0[nb // 2:nb // 2 + 1])
+        l1 = ops.fill_(torch.empty((1, K + 1), dtype=torch.float64, device=dev), 0.0)
         torch.cuda.synchronize()
-        ts = []
-        for _ in range(2):
-            s += 1
-            t1 = time.perf_counter()
-            T1 = hotrg.step2d(T1, chi, d, hotrg.FP64)
-            hotrg.normalize_maxabs(T1, l1, s)
+        t1 = time.perf_counter()
+        hotrg.normalize_maxabs(T1, l1, 0)
+        for n, dd in schedule(K):
+            T1 = hotrg.step2d(T1, chi, dd, hotrg.FP64)
+            hotrg.normalize_maxabs(T1, l1, n)
             torch.cuda.synchronize()
-            ts.append(time.perf_counter() - t1)
-            d = "X" if d == "Y" else "Y"
-        out["plain_fp64_steps_per_s"] = 1.0 / min(ts)
-        out["speedup_vs_plain_fp64"] = value / out["plain_fp64_steps_per_s"]
+        hotrg.lnz2d(T1, l1, K).cpu()
+        tp1 = time.perf_counter() - t1
+        out["plain_fp64"] = {"steps_per_s": K / tp1, "solve_s": tp1, "beta": betas[nb // 2]}
+        out["speedup_vs_plain_fp64"] = value / out["plain_fp64"]["steps_per_s"]
+        log(f"plain float64 solve {tp1:.1f} s")
         del T1
         torch.cuda.empty_cache()
 
-    # ---- accuracy of the mixed policy against the float64 path (untimed) ----
-    if not args.no_validate:
-        vb = [0.40, 0.44]
-        T0 = torch.stack([models.initial_tensor("ising2d", b, dev) for b in vb])
-        Tm, lm = hotrg.renorm2d(T0, chi, 20, 20, policy)
-        zm = hotrg.lnz2d(Tm, lm, 40).cpu().numpy()
-        del Tm
-        Td, ld = hotrg.renorm2d(T0, chi, 20, 20, hotrg.ALL_FP64)
-        zd = hotrg.lnz2d(Td, ld, 40).cpu().numpy()
-        del Td
-        out["lnZ_err_vs_fp64"] = float(np.max(np.abs(zm - zd)))
-        out["lnZ_check"] = {"betas": vb, "mixed": zm.tolist(), "fp64": zd.tolist()}
-        torch.cuda.empty_cache()
-
-    # ---- serving: a short Poisson stream of mixed solves (latency) ----
+    # ---- serving: a Poisson stream of mixed solves (latency) ----
     if not args.no_serve:
         from trg_backup_amd import serve
         reqs = serve.poisson_stream(args.serve_n, 50.0, 2, chis=(16, 24, 32), steps=(20, 30))
@@ -184,16 +411,16 @@ def main():
         done = srv.run(reqs, realtime=True)
         ts = time.perf_counter() - t1
         summ = serve.summarize(done, ts)
-        # float64 references of the 2D (mixed-precision) solves; ATRG flows run float64 throughout
         twod = [f.req for f in done if f.req.model == "ising2d"]
         ref = {f.req.rid: f.result for f in serve.Server(fp64_steps=10 ** 9).run(twod, realtime=False)} if twod else {}
         err = max([abs(f.result - ref[f.req.rid]) for f in done if f.req.rid in ref] or [0.0])
-        out["serving"] = dict(summ, worst_lnZ_err_vs_fp64=err, rejected=len(srv.rejected),
+        out["serving"] = dict(summ, worst_lnZ_err_vs_fp64_2d=err, rejected=len(srv.rejected),
                               stream=f"{args.serve_n} Poisson requests (rate 50/s, seed 2), models "
                                      "{2D Ising HOTRG, 3D Ising ATRG, 3D Potts ATRG}, chi in {16,24,32}, "
                                      "20-30 RG steps, beta within 10% of beta_c")
         out["p50_latency_s"] = summ["p50_latency_s"]
         out["p99_latency_s"] = summ["p99_latency_s"]
+        log(f"serving: {summ}")
         torch.cuda.empty_cache()
 
     # ---- CPU baseline: the NumPy float64 oracle on the host ----
@@ -201,16 +428,16 @@ def main():
         from oracle import HOTRG_2d as oh2
         from oracle import models as omodels
         Tc = omodels.ising2d_tensor(0.44)
-        s, d = 0, "Y"
+        s, dd = 0, "Y"
         while Tc.shape[0] < chi or Tc.shape[2] < chi:
             s += 1
-            Tc, _ = oh2.__tensor_normalization__(oh2.new_pure_tensor(Tc, chi, d))
-            d = "X" if d == "Y" else "Y"
+            Tc, _ = oh2.__tensor_normalization__(oh2.new_pure_tensor(Tc, chi, dd))
+            dd = "X" if dd == "Y" else "Y"
         t1 = time.perf_counter()
-        Tc, _ = oh2.__tensor_normalization__(oh2.new_pure_tensor(Tc, chi, d))
+        Tc, _ = oh2.__tensor_normalization__(oh2.new_pure_tensor(Tc, chi, dd))
         tc = time.perf_counter() - t1
         out["cpu_baseline"] = {"value": 1.0 / tc, "unit": "RG steps/s", "cores": os.cpu_count(), "kind": "port",
-                               "sample": f"one steady-state chi={chi} {d} step of one instance (beta=0.44), "
+                               "sample": f"one steady-state chi={chi} {dd} step of one instance (beta=0.44), "
                                          f"NumPy/OpenBLAS float64 oracle, {tc:.1f} s"}
     print(json.dumps(out), flush=True)
     return 0

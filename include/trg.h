@@ -26,7 +26,10 @@ typedef enum {
     TRG_HIP_ERROR = 6     /* a HIP runtime call failed (see trg_last_hip_error) */
 } trg_status;
 
-typedef enum { TRG_F32 = 0, TRG_F64 = 1, TRG_BF16 = 2 } trg_dtype;
+/* Storage types (F32, F64; I32 for trg_fill only) and GEMM compute types
+ * (F32, F64, BF16, BF16X3 = fp32 operands split into two bf16 parts, three
+ * bf16 MFMAs per product, about 16 significant bits per operand). */
+typedef enum { TRG_F32 = 0, TRG_F64 = 1, TRG_BF16 = 2, TRG_I32 = 3, TRG_BF16X3 = 4 } trg_dtype;
 
 const char *trg_version(void);
 const char *trg_status_string(int status);
@@ -39,8 +42,9 @@ int trg_device_info(int device, int *num_cus, int *lds_per_block, int *warp_size
  * Element (m,k) of A[b] is at A + b*sAb + m*rsA + k*csA (elements, not
  * bytes); likewise B (k,n) and C (m,n).  One of rsA/csA and one of rsB/csB
  * must be 1.  `dtype` is the storage type of A, B and C (F32 or F64);
- * `compute` is F32 or BF16 for F32 storage (BF16: operands rounded to bf16,
- * fp32 accumulation) and F64 for F64 storage.
+ * `compute` is F32, BF16 or BF16X3 for F32 storage (BF16: operands rounded
+ * to bf16, fp32 accumulation; BF16X3: split-bf16 products) and F64 for F64
+ * storage.
  * Replaces every pairwise step of the cuquantum/opt_einsum contractions on
  * the path: trg/HOTRG_2d.py:26-27,65-67; trg/HOTRG_3d.py:42-45,107-116;
  * trg/ATRG_3d_new.py:99-122,232-233,261-265,379-405,424-433.
@@ -79,7 +83,7 @@ int trg_copy_strided(int dtype_in, int dtype_out, int rank, const long long *dim
                      const void *in, const long long *si, void *out, const long long *so,
                      void *stream);
 
-/* out[i] = value for i < n (n elements of dtype). */
+/* out[i] = value for i < n (n elements of dtype: F32, F64 or I32). */
 int trg_fill(int dtype, long long n, double value, void *out, void *stream);
 
 /*

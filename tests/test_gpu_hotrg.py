@@ -106,3 +106,54 @@ def test_hotrg3d_step_matches_oracle_trace(gpu):
         t0, t1 = np.einsum("xxyytt", T0), np.einsum("xxyytt", T1)
         assert abs(t1 - t0) < 1e-11 * abs(t0), (d, t0, t1)
         T, _ = o3.__tensor_normalization__(T0)
+
+
+def test_hotrg2d_chi32_onsager_fp64_and_mixed(gpu):
+    """2D Ising, beta = 0.44, chi = 32, 30 steps: within 1e-6 of Onsager in float64, and the
+    mixed schedule (float64 steps <= 6, float32 after, bf16 merge after step 20) within 1e-8
+    of the float64 result."""
+    from trg_backup_amd import hotrg
+    T0 = torch.as_tensor(models.ising2d_tensor(0.44)).to(gpu)[None]
+    Td, ld = hotrg.renorm2d(T0, 32, 15, 15, hotrg.ALL_FP64)
+    vd = float(hotrg.lnz2d(Td, ld, 30)[0])
+    pol = hotrg.Policy(fp64_steps=6, low=hotrg.FP32, bf16_merge_from=20)
+    Tm, lm = hotrg.renorm2d(T0, 32, 15, 15, pol)
+    vm = float(hotrg.lnz2d(Tm, lm, 30)[0])
+    print(f"\n[hotrg2d chi=32] fp64={vd!r} onsager diff={vd - ONSAGER_044:.3e} mixed diff={vm - vd:.3e}",
+          flush=True)
+    assert abs(vd - ONSAGER_044) < 1e-6
+    assert abs(vm - vd) <= 1e-8
+
+
+def test_hotrg2d_bf16_merge_close_to_fp32(gpu):
+    """One steady chi = 24 step with the merge on bf16 MFMA against the float32 step
+    (trace relative difference at the bf16 rounding level)."""
+    from trg_backup_amd import hotrg, ops
+    T = torch.stack([torch.as_tensor(models.ising2d_tensor(b)) for b in (0.42, 0.45)]).to(gpu)
+    T, _ = hotrg.renorm2d(T, 24, 2, 2, hotrg.ALL_FP64)
+    T = ops.cast(T, torch.float32)
+    a = hotrg.step2d(T, 24, "Y", hotrg.FP32)
+    b = hotrg.step2d(T, 24, "Y", hotrg.Precision(torch.float32, "f32", "bf16"))
+    ta, tb = ops.trace_pairs(a, batch=2).cpu().numpy(), ops.trace_pairs(b, batch=2).cpu().numpy()
+    rel = np.abs(ta - tb) / np.abs(ta)
+    err = float((a.double() - b.double()).norm() / a.double().norm())
+    print(f"\n[bf16 merge] trace rel diff {rel}, tensor rel diff {err:.3e}", flush=True)
+    assert np.all(rel < 2e-2) and err < 2e-2 and err > 0
+
+
+def test_hotrg3d_batched_equals_alone(gpu):
+    """Couplings from default_rng(0).uniform(0.20, 0.24): a batched 3D solve gives each
+    instance exactly its single-instance result (float32 steps after the first three)."""
+    from trg_backup_amd import hotrg
+    betas = np.random.default_rng(0).uniform(0.20, 0.24, 128)[:3]
+    T0 = torch.stack([torch.as_tensor(models.ising3d_tensor(b)) for b in betas]).to(gpu)
+    pol = hotrg.Policy(fp64_steps=3, low=hotrg.FP32)
+    Tb, lb = hotrg.renorm3d(T0, 8, 2, 2, 2, pol)
+    zb = hotrg.lnz3d(Tb, lb, 6).cpu().numpy()
+    for i in range(3):
+        Ti, li = hotrg.renorm3d(T0[i:i + 1], 8, 2, 2, 2, pol)
+        zi = hotrg.lnz3d(Ti, li, 6).cpu().numpy()
+        assert zi[0] == zb[i], (i, zi[0], zb[i])
+    ref = o3.ln_Z_over_V(models.ising3d_tensor(float(betas[0])), 8, 2, 2, 2)
+    print(f"\n[hotrg3d batched chi=8] {zb} oracle[0]={ref!r} diff={zb[0] - ref:.3e}", flush=True)
+    assert abs(zb[0] - ref) < 1e-7

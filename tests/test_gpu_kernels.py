@@ -159,7 +159,8 @@ def test_permute_fast_paths(gpu, perm):
     assert np.array_equal(ys.cpu().numpy(), np.transpose(x[:, 3:30, ::2, :], perm))
 
 
-@pytest.mark.parametrize("dtype,compute", [(torch.float32, None), (torch.float32, "bf16"), (torch.float64, None)])
+@pytest.mark.parametrize("dtype,compute", [(torch.float32, None), (torch.float32, "bf16"), (torch.float32, "bf16x3"),
+                                           (torch.float64, None)])
 @pytest.mark.parametrize("shape,ta,tb", [((300, 260, 100), False, False), ((257, 129, 67), True, False),
                                          ((96, 200, 48), False, True), ((64, 4096, 64), True, True),
                                          ((300, 260, 100), False, True), ((130, 70, 67), False, True),
@@ -178,9 +179,10 @@ def test_gemm_tile_configs_bitwise_equal(gpu, dtype, compute, shape, ta, tb):
     C0 = ops.gemm(Av, Bv, compute=compute)
     ref = np.matmul(A.transpose(0, 2, 1) if ta else A, B.transpose(0, 2, 1) if tb else B)
     err = np.abs(C0.double().cpu().numpy() - ref).max() / (np.abs(A).max() * np.abs(B).max() * K)
-    assert err < (2e-3 if compute == "bf16" else 1e-6 if dtype == torch.float32 else 1e-14), err
+    tol = {"bf16": 2e-3, "bf16x3": 2e-5}.get(compute, 1e-6 if dtype == torch.float32 else 1e-14)
+    assert err < tol, err
     dt = ops.dtype_code(Ad)
-    cc = _lib.BF16 if compute == "bf16" else dt
+    cc = {"bf16": _lib.BF16, "bf16x3": _lib.BF16X3}.get(compute, dt)
     n = autotune.config_count(dt, cc)
     assert n >= 2
     for cfg in range(n):
@@ -214,3 +216,19 @@ def test_gemm_autotuner_picks_a_config_and_keeps_bits(gpu, tmp_path):
     assert autotune.table() == tab and autotune.active()
     assert torch.equal(ops.gemm(A, B), C0)
     autotune.clear()
+
+
+def test_bf16x3_accuracy_between_bf16_and_fp32(gpu):
+    """Split-bf16 products: error far below plain bf16 and within ~2^-16 of the float64 product."""
+    from trg_backup_amd import ops
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((2, 256, 512))
+    B = rng.standard_normal((2, 512, 192))
+    Ad, Bd = _dev(A, gpu).float(), _dev(B, gpu).float()
+    Bt = ops.contiguous(Bd.transpose(1, 2)).transpose(1, 2)        # K-contiguous B
+    ref = A @ B
+    scale = np.abs(ref).max()
+    e = {c: np.abs(ops.gemm(Ad, Bt, compute=c).double().cpu().numpy() - ref).max() / scale
+         for c in (None, "bf16x3", "bf16")}
+    print(f"\n[bf16x3] rel err fp32 {e[None]:.2e} bf16x3 {e['bf16x3']:.2e} bf16 {e['bf16']:.2e}", flush=True)
+    assert e["bf16x3"] < 5e-5 and e["bf16x3"] < e["bf16"] / 50 and e[None] < e["bf16x3"]

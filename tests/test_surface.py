@@ -1,0 +1,67 @@
+"""CPU checks of the drop-in surface: module aliases and the reference's public names.
+
+The name lists are the public functions of the reference modules on the path
+(trg/HOTRG_2d.py:8-300, trg/HOTRG_3d.py:18-398, trg/ATRG_3d_new.py:18-974,
+utility/truncated_svd.py:17-151, utility/randomized_svd.py:5-165,
+tensor_class/tensor_class.py:5-41).
+"""
+import sys
+
+import pytest
+
+SURFACE = {
+    "trg.HOTRG_2d": ["__tensor_legs_rearrange__", "__tensor_legs_restore__", "__contraction_projector__",
+                     "new_pure_tensor", "new_impuer_tensor_2imp", "new_impuer_tensor_2to1imp",
+                     "new_impuer_tensor_1imp", "__tensor_normalization__", "pure_tensor_renorm",
+                     "two_point_func_renorm", "ynearest_two_point_func_renorm"],
+    "trg.HOTRG_3d": ["__tensor_legs_rearrange__", "__tensor_legs_restore__", "__contraction_projector__",
+                     "new_pure_tensor", "new_impuer_tensor_2imp", "new_impuer_tensor_2to1imp",
+                     "new_impuer_tensor_1imp", "__tensor_normalization__", "pure_tensor_renorm",
+                     "two_point_func_renorm", "ynearest_two_point_func_renorm"],
+    "trg.ATRG_3d_new": ["leg_transposition", "rsvd_for_ATRG_leg_swapping", "leg_swapping", "squeezer",
+                        "leg_slicing", "coarse_graining", "atrg_pure_tensor", "normalization",
+                        "exec_pure_tensor_renorm", "pure_tensor"],
+    "utility.truncated_svd": ["truncate", "svd", "eigh"],
+    "utility.randomized_svd": ["rsvd", "rsvd2", "__power_iteration__", "__QR_iteration__",
+                               "rsvd_for_3dATRG_tensor_init", "rsvd_for_3dATRG_tensor_inte"],
+    "tensor_class.tensor_class": ["HOTRG_Tensor", "Tensor", "ATRG_Tensor"],
+}
+
+
+@pytest.fixture
+def aliases():
+    from trg_backup_amd import dropin
+    saved = {k: sys.modules.pop(k) for k in list(dropin.ALIASES) if k in sys.modules}
+    names = dropin.install()
+    yield names
+    dropin.uninstall()
+    sys.modules.update(saved)
+
+
+def test_aliases_resolve_to_the_gpu_modules(aliases):
+    import importlib
+    assert set(aliases) == set(SURFACE) | {"trg", "utility", "tensor_class"}
+    for alias in SURFACE:
+        mod = importlib.import_module(alias)
+        assert mod.__name__ == "trg_backup_amd." + alias
+
+
+@pytest.mark.parametrize("alias", sorted(SURFACE))
+def test_reference_names_exist(aliases, alias):
+    import importlib
+    mod = importlib.import_module(alias)
+    missing = [n for n in SURFACE[alias] if not hasattr(mod, n)]
+    assert not missing, missing
+
+
+def test_numpy_input_without_library_or_gpu_fails_loudly(monkeypatch):
+    """No CPU fallback: with the library missing, a call raises TrgError."""
+    import numpy as np
+
+    from trg_backup_amd import _lib
+    from trg_backup_amd.utility import truncated_svd
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libtrg.so")
+    with pytest.raises(Exception) as e:
+        truncated_svd.svd(np.eye(4), shape=[[0], [1]], k=2)
+    assert isinstance(e.value, (_lib.TrgError, RuntimeError, AssertionError))

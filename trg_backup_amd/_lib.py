@@ -10,7 +10,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtrg.so")
 
-F32, F64, BF16 = 0, 1, 2
+F32, F64, BF16, I32, BF16X3 = 0, 1, 2, 3, 4
 
 c_int, c_ll, c_dbl, c_vp, c_ull, c_uint = (ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p,
                                            ctypes.c_ulonglong, ctypes.c_uint)

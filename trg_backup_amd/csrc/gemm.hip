@@ -257,8 +257,60 @@ gemm_kernel(GemmArgs g) {
 // fp32 storage, bf16 MFMA.  LDS holds As[m][k] and Bs[n][k] in bf16 (k
 // fastest) because v_mfma_f32_32x32x16_bf16 takes 8 consecutive k per lane:
 // lane holds A[m = lane % 32][k = 8 * (lane / 32) + j], j < 8.
+//
+// SPLIT ("bf16x3"): every fp32 operand x is staged as hi = bf16(x) and
+// lo = bf16(x - hi) (the subtraction is exact), and each product is formed
+// as lo_a hi_b + hi_a lo_b + hi_a hi_b on the MFMA (the lo*lo term, below
+// 2^-16 relative, is dropped).  That carries about 16 significant bits per
+// operand instead of 8, at three bf16 MFMAs per block -- still far fewer
+// matrix-core cycles than one fp32 MFMA chain over the same K.
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int BK, int WM, int WN, bool A_MCONT, bool B_NCONT>
+__device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+// One BK slice of MFMAs from the k-fastest LDS tiles (hi parts at offset 0,
+// lo parts after BM * KP / BN * KP elements when SPLIT).
+template <int BM, int BN, int BK, int KP, int WM, int WN, bool SPLIT>
+__device__ inline void bf16_slice(const __bf16 *As, const __bf16 *Bs, f32x16 (&acc)[WM / 32][WN / 32], int wm,
+                                  int wn, int lane) {
+    constexpr int TM = WM / 32, TN = WN / 32;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+        bf16x8 a[TM], bb[TN];
+        const int kr = kk + 8 * (lane >> 5);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+            a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm * WM + i * 32 + (lane & 31)) * KP + kr]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+            bb[j] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn * WN + j * 32 + (lane & 31)) * KP + kr]);
+        if constexpr (SPLIT) {
+            bf16x8 al[TM], bl[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+                al[i] = *reinterpret_cast<const bf16x8 *>(&As[BM * KP + (wm * WM + i * 32 + (lane & 31)) * KP + kr]);
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                bl[j] = *reinterpret_cast<const bf16x8 *>(&Bs[BN * KP + (wn * WN + j * 32 + (lane & 31)) * KP + kr]);
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bb[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bl[j], acc[i][j], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
+    }
+}
+
+template <int BM, int BN, int BK, int WM, int WN, bool A_MCONT, bool B_NCONT, bool SPLIT>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
 gemm_bf16_kernel(GemmArgs g) {
     constexpr int WAVES_M = BM / WM;
@@ -268,10 +320,11 @@ gemm_bf16_kernel(GemmArgs g) {
     constexpr int KP = BK + 8;  // row pitch in bf16 elements (16-byte aligned rows)
     constexpr int AE = BM * BK / NT;
     constexpr int BE = BN * BK / NT;
+    constexpr int NS = SPLIT ? 2 : 1;
     static_assert(BK % 16 == 0, "BK must be a multiple of 16");
 
-    __shared__ __attribute__((aligned(16))) __bf16 As[BM * KP];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[BN * KP];
+    __shared__ __attribute__((aligned(16))) __bf16 As[NS * BM * KP];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[NS * BN * KP];
 
     int tm, tn;
     tile_coords(g, tm, tn);
@@ -317,14 +370,16 @@ gemm_bf16_kernel(GemmArgs g) {
             const int e = tid + i * NT;
             int mm, kk;
             if (A_MCONT) { kk = e / BM; mm = e % BM; } else { mm = e / BK; kk = e % BK; }
-            As[mm * KP + kk] = (__bf16)ra[i];
+            if constexpr (SPLIT) split_bf16(ra[i], As[mm * KP + kk], As[BM * KP + mm * KP + kk]);
+            else As[mm * KP + kk] = (__bf16)ra[i];
         }
 #pragma unroll
         for (int i = 0; i < BE; ++i) {
             const int e = tid + i * NT;
             int nn, kk;
             if (B_NCONT) { kk = e / BN; nn = e % BN; } else { nn = e / BK; kk = e % BK; }
-            Bs[nn * KP + kk] = (__bf16)rb[i];
+            if constexpr (SPLIT) split_bf16(rb[i], Bs[nn * KP + kk], Bs[BN * KP + nn * KP + kk]);
+            else Bs[nn * KP + kk] = (__bf16)rb[i];
         }
     };
 
@@ -334,22 +389,7 @@ gemm_bf16_kernel(GemmArgs g) {
     for (int k0 = 0; k0 < g.K; k0 += BK) {
         const bool more = k0 + BK < g.K;
         if (more) load(k0 + BK);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 16) {
-            bf16x8 a[TM], bb[TN];
-            const int kr = kk + 8 * (lane >> 5);
-#pragma unroll
-            for (int i = 0; i < TM; ++i)
-                a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm * WM + i * 32 + (lane & 31)) * KP + kr]);
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
-                bb[j] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn * WN + j * 32 + (lane & 31)) * KP + kr]);
-#pragma unroll
-            for (int i = 0; i < TM; ++i)
-#pragma unroll
-                for (int j = 0; j < TN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
-        }
+        bf16_slice<BM, BN, BK, KP, WM, WN, SPLIT>(As, Bs, acc, wm, wn, lane);
         __syncthreads();
         if (more) {
             store();
@@ -389,7 +429,7 @@ gemm_bf16_kernel(GemmArgs g) {
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-template <int BM, int BN, int BK, int WM, int WN>
+template <int BM, int BN, int BK, int WM, int WN, bool SPLIT>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
 gemm_bf16k_kernel(GemmArgs g) {
     constexpr int WAVES_M = BM / WM;
@@ -400,11 +440,12 @@ gemm_bf16k_kernel(GemmArgs g) {
     constexpr int KQ = BK / 4;            // fp32 vectors per tile row
     constexpr int AV = BM * KQ / NT;
     constexpr int BV = BN * KQ / NT;
+    constexpr int NS = SPLIT ? 2 : 1;
     static_assert(AV * NT == BM * KQ && BV * NT == BN * KQ, "tile not divisible by threads");
     static_assert(BK % 16 == 0, "BK must be a multiple of 16");
 
-    __shared__ __attribute__((aligned(16))) __bf16 As[BM * KP];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[BN * KP];
+    __shared__ __attribute__((aligned(16))) __bf16 As[NS * BM * KP];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[NS * BN * KP];
 
     int tm, tn;
     tile_coords(g, tm, tn);
@@ -442,22 +483,29 @@ gemm_bf16k_kernel(GemmArgs g) {
             else rb[i] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
     };
+    auto put = [&](__bf16 *S, int rows, int off, const f32x4 &x) {
+        bf16x4 hi, lo;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            __bf16 h, l;
+            if constexpr (SPLIT) split_bf16(x[t], h, l);
+            else h = (__bf16)x[t];
+            hi[t] = h;
+            if constexpr (SPLIT) lo[t] = l;
+        }
+        *reinterpret_cast<bf16x4 *>(&S[off]) = hi;
+        if constexpr (SPLIT) *reinterpret_cast<bf16x4 *>(&S[rows * KP + off]) = lo;
+    };
     auto store = [&]() {
 #pragma unroll
         for (int i = 0; i < AV; ++i) {
             const int e = tid + i * NT;
-            bf16x4 v;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) v[t] = (__bf16)ra[i][t];
-            *reinterpret_cast<bf16x4 *>(&As[(e / KQ) * KP + 4 * (e % KQ)]) = v;
+            put(As, BM, (e / KQ) * KP + 4 * (e % KQ), ra[i]);
         }
 #pragma unroll
         for (int i = 0; i < BV; ++i) {
             const int e = tid + i * NT;
-            bf16x4 v;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) v[t] = (__bf16)rb[i][t];
-            *reinterpret_cast<bf16x4 *>(&Bs[(e / KQ) * KP + 4 * (e % KQ)]) = v;
+            put(Bs, BN, (e / KQ) * KP + 4 * (e % KQ), rb[i]);
         }
     };
 
@@ -467,22 +515,7 @@ gemm_bf16k_kernel(GemmArgs g) {
     for (int k0 = 0; k0 < g.K; k0 += BK) {
         const bool more = k0 + BK < g.K;
         if (more) load(k0 + BK);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 16) {
-            bf16x8 a[TM], bb[TN];
-            const int kr = kk + 8 * (lane >> 5);
-#pragma unroll
-            for (int i = 0; i < TM; ++i)
-                a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm * WM + i * 32 + (lane & 31)) * KP + kr]);
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
-                bb[j] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn * WN + j * 32 + (lane & 31)) * KP + kr]);
-#pragma unroll
-            for (int i = 0; i < TM; ++i)
-#pragma unroll
-                for (int j = 0; j < TN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
-        }
+        bf16_slice<BM, BN, BK, KP, WM, WN, SPLIT>(As, Bs, acc, wm, wn, lane);
         __syncthreads();
         if (more) {
             store();
@@ -551,26 +584,33 @@ static bool kvec_ok(const GemmArgs &g, bool amc, bool bnc) {
 }
 
 // KONLY: the tile exists for the K-contiguous kernel only (its generic
-// counterpart would spill registers); other layouts run tile 256x128x32 w64x64,
-// which gives the same bits.
-template <int BM, int BN, int BK, int WM, int WN, bool KONLY = false>
+// counterpart would spill registers); other layouts run tile 256x128x32 w64x64
+// (128x128x32 w64x64 when SPLIT), which gives the same bits.
+template <int BM, int BN, int BK, int WM, int WN, bool SPLIT, bool KONLY = false>
 static int launch_bf16(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
     if constexpr (KONLY) {
-        if (!kvec_ok(g, amc, bnc)) return launch_bf16<256, 128, 32, 64, 64>(g, batch, amc, bnc, s);
+        if (!kvec_ok(g, amc, bnc)) {
+            if constexpr (SPLIT) return launch_bf16<128, 128, 32, 64, 64, true>(g, batch, amc, bnc, s);
+            else return launch_bf16<256, 128, 32, 64, 64, false>(g, batch, amc, bnc, s);
+        }
     }
     g.tiles_m = (int)cdiv(g.M, BM);
     g.tiles_n = (int)cdiv(g.N, BN);
     dim3 grid(g.tiles_m * g.tiles_n, batch);
     dim3 block((BM / WM) * (BN / WN) * 64);
     if (KONLY || kvec_ok(g, amc, bnc)) {
-        hipLaunchKernelGGL((gemm_bf16k_kernel<BM, BN, BK, WM, WN>), grid, block, 0, s, g);
+        hipLaunchKernelGGL((gemm_bf16k_kernel<BM, BN, BK, WM, WN, SPLIT>), grid, block, 0, s, g);
         return check_launch();
     }
     if constexpr (!KONLY) {
-    if (amc && bnc) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, true>), grid, block, 0, s, g);
-    else if (amc) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, false>), grid, block, 0, s, g);
-    else if (bnc) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, true>), grid, block, 0, s, g);
-    else hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, false>), grid, block, 0, s, g);
+        if (amc && bnc)
+            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, true, SPLIT>), grid, block, 0, s, g);
+        else if (amc)
+            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, false, SPLIT>), grid, block, 0, s, g);
+        else if (bnc)
+            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, true, SPLIT>), grid, block, 0, s, g);
+        else
+            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, false, SPLIT>), grid, block, 0, s, g);
     }
     return check_launch();
 }
@@ -603,16 +643,28 @@ static int gemm_dispatch(int dtype, int compute, int config, GemmArgs g, int bat
     if (dtype == TRG_F32 && compute == TRG_BF16) {
         if (config < 0) config = work >= 256LL * 128 * 128 ? 0 : 1;
         switch (config) {
-            case 0: return launch_bf16<128, 128, 32, 64, 64>(g, batch, amc, bnc, s);
-            case 1: return launch_bf16<64, 64, 32, 32, 32>(g, batch, amc, bnc, s);
-            case 2: return launch_bf16<256, 128, 32, 64, 64>(g, batch, amc, bnc, s);
-            case 3: return launch_bf16<256, 128, 32, 128, 64>(g, batch, amc, bnc, s);
-            case 4: return launch_bf16<128, 256, 32, 64, 128>(g, batch, amc, bnc, s);
-            case 5: return launch_bf16<128, 128, 64, 64, 64>(g, batch, amc, bnc, s);
-            case 6: return launch_bf16<256, 128, 64, 128, 64, true>(g, batch, amc, bnc, s);
-            case 7: return launch_bf16<256, 256, 32, 128, 64, true>(g, batch, amc, bnc, s);
-            case 8: return launch_bf16<256, 256, 16, 128, 64>(g, batch, amc, bnc, s);
-            case 9: return launch_bf16<128, 256, 32, 64, 64>(g, batch, amc, bnc, s);
+            case 0: return launch_bf16<128, 128, 32, 64, 64, false>(g, batch, amc, bnc, s);
+            case 1: return launch_bf16<64, 64, 32, 32, 32, false>(g, batch, amc, bnc, s);
+            case 2: return launch_bf16<256, 128, 32, 64, 64, false>(g, batch, amc, bnc, s);
+            case 3: return launch_bf16<256, 128, 32, 128, 64, false>(g, batch, amc, bnc, s);
+            case 4: return launch_bf16<128, 256, 32, 64, 128, false>(g, batch, amc, bnc, s);
+            case 5: return launch_bf16<128, 128, 64, 64, 64, false>(g, batch, amc, bnc, s);
+            case 6: return launch_bf16<256, 128, 64, 128, 64, false, true>(g, batch, amc, bnc, s);
+            case 7: return launch_bf16<256, 256, 32, 128, 64, false, true>(g, batch, amc, bnc, s);
+            case 8: return launch_bf16<256, 256, 16, 128, 64, false>(g, batch, amc, bnc, s);
+            case 9: return launch_bf16<128, 256, 32, 64, 64, false>(g, batch, amc, bnc, s);
+            default: return TRG_BAD_ARG;
+        }
+    }
+    if (dtype == TRG_F32 && compute == TRG_BF16X3) {
+        if (config < 0) config = work >= 256LL * 128 * 128 ? 0 : 1;
+        switch (config) {
+            case 0: return launch_bf16<128, 128, 32, 64, 64, true>(g, batch, amc, bnc, s);
+            case 1: return launch_bf16<64, 64, 32, 32, 32, true>(g, batch, amc, bnc, s);
+            case 2: return launch_bf16<256, 128, 32, 64, 64, true>(g, batch, amc, bnc, s);
+            case 3: return launch_bf16<128, 256, 32, 64, 64, true>(g, batch, amc, bnc, s);
+            case 4: return launch_bf16<256, 256, 32, 128, 64, true, true>(g, batch, amc, bnc, s);
+            case 5: return launch_bf16<256, 128, 32, 128, 64, true, true>(g, batch, amc, bnc, s);
             default: return TRG_BAD_ARG;
         }
     }
@@ -659,6 +711,7 @@ extern "C" int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, 
 extern "C" int trg_gemm_config_count(int dtype, int compute) {
     if (dtype == TRG_F32 && compute == TRG_F32) return 8;
     if (dtype == TRG_F32 && compute == TRG_BF16) return 10;
+    if (dtype == TRG_F32 && compute == TRG_BF16X3) return 6;
     if (dtype == TRG_F64 && compute == TRG_F64) return 4;
     return 0;
 }
@@ -669,10 +722,13 @@ extern "C" const char *trg_gemm_config_name(int dtype, int compute, int config) 
     static const char *bf[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "256x128x32 w128x64",
                                "128x256x32 w64x128", "128x128x64 w64x64", "256x128x64 w128x64",
                                "256x256x32 w128x64", "256x256x16 w128x64", "128x256x32 w64x64"};
+    static const char *bx3[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "128x256x32 w64x64",
+                                "256x256x32 w128x64", "256x128x32 w128x64"};
     static const char *f64[] = {"128x128x16 w64x64", "64x64x16 w32x32", "128x64x16 w64x32", "64x128x16 w32x64"};
     if (config < 0 || config >= trg_gemm_config_count(dtype, compute)) return "";
     if (dtype == TRG_F32 && compute == TRG_F32) return f32[config];
     if (dtype == TRG_F32 && compute == TRG_BF16) return bf[config];
+    if (dtype == TRG_F32 && compute == TRG_BF16X3) return bx3[config];
     return f64[config];
 }
 

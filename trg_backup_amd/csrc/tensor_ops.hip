@@ -417,7 +417,7 @@ extern "C" int trg_fill(int dtype, long long n, double value, void *out, void *s
     hipStream_t s = as_stream(stream);
     if (dtype == TRG_F32) hipLaunchKernelGGL(fill_kernel<float>, grid_for(n, 256), 256, 0, s, (float *)out, n, (float)value);
     else if (dtype == TRG_F64) hipLaunchKernelGGL(fill_kernel<double>, grid_for(n, 256), 256, 0, s, (double *)out, n, value);
-    else if (dtype == 3) hipLaunchKernelGGL(fill_kernel<int>, grid_for(n, 256), 256, 0, s, (int *)out, n, (int)value);
+    else if (dtype == TRG_I32) hipLaunchKernelGGL(fill_kernel<int>, grid_for(n, 256), 256, 0, s, (int *)out, n, (int)value);
     else return TRG_UNSUPPORTED;
     return check_launch();
 }

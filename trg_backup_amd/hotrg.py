@@ -34,17 +34,18 @@ MERGE_BUDGET = 32 << 30
 @dataclass(frozen=True)
 class Precision:
     """Storage dtype of T and the GEMM compute type of the environment and
-    the merge ("f64", "f32", "bf16").  Eigensolvers, traces and log-norms
-    are always float64."""
+    the merge ("f64", "f32", "bf16x3" = split-bf16 products with about 16
+    significant bits per operand, "bf16").  Eigensolvers, traces and
+    log-norms are always float64."""
     store: torch.dtype = torch.float64
     env: str = "f64"
     merge: str = "f64"
 
     def env_compute(self):
-        return None if self.store == torch.float64 else ("bf16" if self.env == "bf16" else None)
+        return None if self.store == torch.float64 else (self.env if self.env in ("bf16", "bf16x3") else None)
 
     def merge_compute(self):
-        return None if self.store == torch.float64 else ("bf16" if self.merge == "bf16" else None)
+        return None if self.store == torch.float64 else (self.merge if self.merge in ("bf16", "bf16x3") else None)
 
 
 FP64 = Precision()
@@ -57,11 +58,13 @@ class Policy:
     weight 2^-n (lnf[n] = ln c_n / 2^n and T_n only feeds later terms), so
     early steps run in float64 and later ones in lower precision."""
 
-    def __init__(self, fp64_steps=10**9, low=FP32, bf16_env_from=None, bf16_merge_from=None):
+    def __init__(self, fp64_steps=10**9, low=FP32, bf16_env_from=None, bf16_merge_from=None,
+                 split_merge_from=None):
         self.fp64_steps = fp64_steps
         self.low = low
         self.bf16_env_from = bf16_env_from
         self.bf16_merge_from = bf16_merge_from
+        self.split_merge_from = split_merge_from
 
     def __call__(self, step):
         if step <= self.fp64_steps:
@@ -69,6 +72,8 @@ class Policy:
         env, merge = self.low.env, self.low.merge
         if self.bf16_env_from is not None and step > self.bf16_env_from:
             env = "bf16"
+        if self.split_merge_from is not None and step > self.split_merge_from:
+            merge = "bf16x3"
         if self.bf16_merge_from is not None and step > self.bf16_merge_from:
             merge = "bf16"
         if (env, merge) == (self.low.env, self.low.merge):
@@ -79,6 +84,8 @@ class Policy:
         s = f"fp64 for steps <= {self.fp64_steps}, then {self.low.store} env={self.low.env} merge={self.low.merge}"
         if self.bf16_env_from is not None:
             s += f", bf16 env after step {self.bf16_env_from}"
+        if self.split_merge_from is not None:
+            s += f", bf16x3 merge after step {self.split_merge_from}"
         if self.bf16_merge_from is not None:
             s += f", bf16 merge after step {self.bf16_merge_from}"
         return s
@@ -186,7 +193,7 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
         return res
     # fp32 MFMA wants K-outer operands, bf16 MFMA K-contiguous ones (8
     # consecutive k per lane); the permutations below produce either layout.
-    kinner = compute == "bf16"
+    kinner = compute in ("bf16", "bf16x3")
     if kinner:
         Tp = permute(T2, (0, 2, 4, 3, 1)).view(B, Dx * Dy, Dy * Dx).transpose(1, 2)   # [(e,b),(d,l)]
     else:

@@ -6,7 +6,7 @@ arithmetic operation below is a libtrg kernel.
 import torch
 
 from . import _lib, autotune
-from ._lib import BF16, F32, F64, call, ll_array
+from ._lib import BF16, BF16X3, F32, F64, I32, call, ll_array
 
 _DT = {torch.float32: F32, torch.float64: F64}
 
@@ -91,7 +91,7 @@ def fill_(t, value):
     require_device(t)
     if not t.is_contiguous():
         raise _lib.TrgError("fill_ needs a contiguous tensor")
-    code = 3 if t.dtype == torch.int32 else dtype_code(t)
+    code = I32 if t.dtype == torch.int32 else dtype_code(t)
     call("trg_fill", code, t.numel(), float(value), ptr(t), stream_ptr())
     return t
 
@@ -106,7 +106,7 @@ class GemmRecorder:
     def __init__(self, tag):
         self.tag = tag
         self.events = []   # (start, end, flops)
-        self.configs = set()   # (dtype, compute, tile config, M, N, K, batch) launched
+        self.event_configs = []   # (dtype, compute, tile config, M, N, K, batch) per event
 
     def __enter__(self):
         _RECORDERS.append(self)
@@ -134,7 +134,7 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None):
 
     A is (M, K) or (batch, M, K), B is (K, N) or (batch, K, N).  Views may
     be transposed as long as one of the two matrix strides of A and of B is 1.
-    compute: None (same as storage), "bf16" (f32 storage only) or a dtype code.
+    compute: None (same as storage), "bf16" / "bf16x3" (f32 storage only) or a dtype code.
     """
     require_device(A, B)
     batched = A.dim() == 3
@@ -174,6 +174,8 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None):
         cc = dt
     elif compute == "bf16":
         cc = BF16
+    elif compute == "bf16x3":
+        cc = BF16X3
     elif compute == "f32":
         cc = F32
     elif compute == "f64":
@@ -200,7 +202,7 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None):
         ev1.record()
         for r in recs:
             r.events.append((ev0, ev1, 2.0 * M * N * K * batch))
-            r.configs.add((dt, cc, config, M, N, K, batch))
+            r.event_configs.append((dt, cc, config, M, N, K, batch))
     return C if batched else C[0]
 
 

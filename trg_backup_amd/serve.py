@@ -7,12 +7,14 @@ number of RG steps.  The server
     footprints of the admitted flows stays within the budget (requests whose
     footprint alone exceeds the budget are rejected);
   * keeps every admitted flow's tensors resident on the GPU between steps;
-  * advances all active flows by one RG step per tick: 2D HOTRG flows that
-    are at the same step shape and precision are stacked into one batched
-    launch sequence; ATRG flows (dynamic bonds) step individually; the groups
-    of a tick run concurrently on a small pool of HIP streams (one host
-    thread each), so the work of different requests overlaps on the device;
-  * admits new arrivals at every tick, as memory and slots free up;
+  * runs RG steps on a small pool of HIP streams (one host thread each):
+    whenever a stream is free it takes the oldest ready flow together with
+    every other ready 2D HOTRG flow at the same step shape, direction and
+    precision, stacked into one batched launch sequence (ATRG flows, whose
+    bonds are dynamic, step individually); a flow is ready again as soon as
+    its step has finished, so the work of different requests overlaps on
+    the device and short flows are not held back by long ones;
+  * admits new arrivals whenever they come, as memory frees up;
   * records the latency arrival -> result of every request.
 
 Every kernel is batch-invariant (an instance's arithmetic does not depend on
@@ -25,11 +27,10 @@ measurement/Ising_model_3d.py:114-122, measurement/Potts3_model_3d.py:32-40),
 because 3D HOTRG needs chi^6 storage and 2 chi^11 flop per step (275 GB and
 1.5e20 flop at chi = 64).
 """
-import itertools
 import math
 import threading
 import time
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -81,7 +82,7 @@ def footprint(req, policy=None):
 
 
 class Server:
-    def __init__(self, hbm_budget=None, max_batch=32, streams=2, fp64_steps=10, atrg_q=2, atrg_deg_eps=1e-8):
+    def __init__(self, hbm_budget=None, max_batch=32, streams=3, fp64_steps=10, atrg_q=2, atrg_deg_eps=1e-8):
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.6 * free)
@@ -117,6 +118,11 @@ class Server:
         return "TXY"[(step - 1) % 3]                # trg/ATRG_3d_new.py:999-1000
 
     def _step_group(self, flows, stream):
+        """One RG step of every flow in `flows` on `stream` (a worker thread).
+        The tensors a step replaces stay referenced until the stream has
+        drained: the caching allocator may hand their blocks to another
+        stream as soon as they are released."""
+        keep = []
         with torch.cuda.stream(stream):
             if flows[0].scheme == "hotrg2d":
                 n0 = flows[0].step + 1
@@ -134,18 +140,22 @@ class Server:
                     n = f.step + 1
                     Ti = ops.contiguous(T[i:i + 1])
                     ops.normalize_(Ti, c[i:i + 1], f.lnf, n, batch=1, lnf_stride=f.lnf.shape[1])
+                    keep.append(f.T)
                     f.T = Ti
                     f.step = n
+                keep += [T, c]
             else:
-                old = atrg.degeneracy_eps
                 for f in flows:
                     n = f.step + 1
                     U, s, VH = atrg.atrg_pure_tensor(f.U, f.s, f.VH, f.req.chi, self._atrg_direction(n), "f64")
                     s, c = atrg.normalize(U, s, VH)
+                    keep.append((f.U, f.s, f.VH))
                     f.U, f.s, f.VH = U, s, VH
                     f.consts.append(c)
                     f.step = n
             stream.synchronize()
+        del keep
+        return flows
 
     def _finish(self, fl, now):
         r = fl.req
@@ -172,16 +182,37 @@ class Server:
             atrg.RSVD.update(atrg_params)
             atrg.degeneracy_eps = old_deg
 
+    def _group_key(self, f):
+        if f.scheme == "hotrg2d":
+            n = f.step + 1
+            return ("h", f.req.chi, tuple(f.T.shape[1:]), f.T.dtype, self._hotrg_direction(n), self.policy(n))
+        return ("a", f.req.rid)
+
+    def _take_group(self, ready):
+        """The oldest ready flow, with every other ready 2D flow that can share
+        its launches (same step shape, direction and precision)."""
+        key = self._group_key(ready[0])
+        group, rest = [], []
+        for f in ready:
+            if len(group) < self.max_batch and self._group_key(f) == key:
+                group.append(f)
+            else:
+                rest.append(f)
+        ready[:] = rest
+        return group
+
     def _run(self, queue, realtime, log):
+        """Event-driven loop: a flow goes back to the ready list as soon as its
+        step has finished, and a free stream takes the next ready group at once,
+        so fast flows are never held back by slow ones."""
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
-        active, done, used = [], [], 0
-        qi = 0
-        while qi < len(queue) or active:
-            # admission in arrival order, by HBM footprint
+        ready, running, done, used, qi = [], {}, [], 0, 0
+        free = list(range(len(self.streams)))
+        while qi < len(queue) or ready or running:
+            admitted = False
             while qi < len(queue) and (not realtime or queue[qi].arrival <= now()):
                 r = queue[qi]
-                scheme = "hotrg2d" if r.model == "ising2d" else "atrg3d"
                 fp = footprint(r)
                 if fp > self.budget:
                     self.rejected.append(r)
@@ -189,48 +220,41 @@ class Server:
                     continue
                 if used + fp > self.budget:
                     break
-                fl = Flow(req=r, scheme=scheme, footprint=fp, admitted_at=now())
+                fl = Flow(req=r, scheme="hotrg2d" if r.model == "ising2d" else "atrg3d", footprint=fp,
+                          admitted_at=now())
                 self._start(fl)
-                active.append(fl)
+                ready.append(fl)
                 used += fp
                 self.max_used = max(self.max_used, used)
+                admitted = True
                 qi += 1
-            if not active:
+            if admitted:
+                # set-up kernels ran on this thread's stream; the workers' streams must see them done
+                torch.cuda.current_stream().synchronize()
+            while free and ready:
+                group = self._take_group(ready)
+                si = free.pop(0)
+                running[self.pool.submit(self._step_group, group, self.streams[si])] = (group, si)
+            if not running:
                 if qi < len(queue) and realtime:
                     time.sleep(max(0.0, min(0.01, queue[qi].arrival - now())))
                 continue
-            # set-up work of newly admitted flows ran on this thread's stream;
-            # the worker streams must see it complete
-            torch.cuda.current_stream().synchronize()
-            # one RG step for every active flow: group 2D flows by step shape and precision
-            groups = {}
-            for f in active:
-                if f.scheme == "hotrg2d":
-                    n = f.step + 1
-                    key = ("h", f.req.chi, tuple(f.T.shape[1:]), f.T.dtype, self._hotrg_direction(n), self.policy(n))
-                else:
-                    key = ("a", f.req.rid)
-                groups.setdefault(key, []).append(f)
-            batches = []
-            for key, fl in groups.items():
-                for i in range(0, len(fl), self.max_batch):
-                    batches.append(fl[i:i + self.max_batch])
-            futs = [self.pool.submit(self._step_group, b, self.streams[i % len(self.streams)])
-                    for i, b in enumerate(batches)]
-            for fu in futs:
+            timeout = max(0.0, queue[qi].arrival - now()) if (realtime and qi < len(queue)) else None
+            finished, _ = wait(list(running), timeout=timeout, return_when=FIRST_COMPLETED)
+            for fu in finished:
+                group, si = running.pop(fu)
                 fu.result()
-            still = []
-            for f in active:
-                if f.step >= f.req.steps:
-                    self._finish(f, now())
-                    used -= f.footprint
-                    done.append(f)
-                    if log:
-                        log(f"done rid={f.req.rid} {f.req.model} chi={f.req.chi} steps={f.req.steps} "
-                            f"lnZ/V={f.result:.12f} latency={f.finished_at - f.req.arrival:.3f}s")
-                else:
-                    still.append(f)
-            active = still
+                free.append(si)
+                for f in group:
+                    if f.step >= f.req.steps:
+                        self._finish(f, now())
+                        used -= f.footprint
+                        done.append(f)
+                        if log:
+                            log(f"done rid={f.req.rid} {f.req.model} chi={f.req.chi} steps={f.req.steps} "
+                                f"lnZ/V={f.result:.12f} latency={f.finished_at - f.req.arrival:.3f}s")
+                    else:
+                        ready.append(f)
         return done
 
 
