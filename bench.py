@@ -8,10 +8,11 @@ the schedule (K = 40 by default: bond growth, float64 steps, float32 steps, bf16
 with the inputs resident in HBM, then ln Z/V, closed by a device synchronisation.
 value = 32 * K / (timed seconds)  [RG steps/s, whole GPU].
 
-Precision schedule (hotrg.Policy): steps 1..F in float64 (F = --fp64-steps), then float32 on
-MFMA with float64 isometry eigensolvers, traces and log-norms; after step --bf16-merge-from the
-chi^7 merge GEMM runs on bf16 MFMA with float32 accumulation.  An error made at step n enters
-ln Z/V with weight 2^-n.
+Precision schedule (hotrg.Policy): steps 1..6 in float64, then float32 storage with float64
+isometry eigensolvers, traces and log-norms; the merge GEMMs run on float32 MFMA up to step 10,
+on split-bf16 ("bf16x3", about 16 significant bits per operand) up to step 22 and on bf16 after
+that, the environment GEMMs on bf16 after step 12.  An error made at step n enters ln Z/V with
+weight 2^-n, which is what lets later steps run at lower precision.
 
 Warm-up (untimed): the same schedule with every precision tier after step 6 cut to its first
 W steps, which loads every kernel and lets the GEMM autotuner time each tile configuration on
@@ -38,15 +39,17 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"f32": 157.3, "bf16": 2500.0, "f64": 78.6}   # MI355X dense MFMA peaks
+# MI355X dense MFMA peaks; split-bf16 issues three bf16 MFMAs per product, so its peak in
+# algorithmic (fp32-equivalent) flop/s is a third of the bf16 one
+PEAK_TFLOPS = {"f32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0 / 3, "f64": 78.6}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="hotrg2d", choices=["hotrg2d", "hotrg3d", "atrg3d"],
+    ap.add_argument("--workload", default="hotrg2d", choices=["hotrg2d", "hotrg3d", "atrg3d", "serve"],
                     help="hotrg2d: the headline line; hotrg3d / atrg3d: the batched 3D Ising HOTRG and the "
-                         "3D q=3 Potts ATRG configurations (secondary lines)")
+                         "3D q=3 Potts ATRG configurations; serve: a Poisson request stream (secondary lines)")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed RG steps of the schedule (hotrg2d 40 = whole solve; hotrg3d 6; atrg3d 30)")
     ap.add_argument("--warmup", type=int, default=2, help="warm-up steps per precision tier")
@@ -55,7 +58,9 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=None, help="hotrg2d 6, hotrg3d 3")
     ap.add_argument("--atrg-q", type=int, default=2)
     ap.add_argument("--atrg-prec", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--bf16-merge-from", type=int, default=20, help="bf16 merge after this step (-1: never)")
+    ap.add_argument("--split-merge-from", type=int, default=10, help="bf16x3 merge after this step (-1: never)")
+    ap.add_argument("--bf16-env-from", type=int, default=12, help="bf16 environments after this step (-1: never)")
+    ap.add_argument("--bf16-merge-from", type=int, default=22, help="bf16 merge after this step (-1: never)")
     ap.add_argument("--validate-betas", type=int, default=4)
     ap.add_argument("--validate-all", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
@@ -63,9 +68,11 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-n", type=int, default=24)
+    ap.add_argument("--serve-chis", default="16,24,32", help="bond dimensions of the request stream")
+    ap.add_argument("--serve-steps", default="20,30", help="RG-step range of the request stream")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true",
-                    help="run one float32 and one bf16-merge steady step plus a 1 GiB copy of known size "
+                    help="run one steady step with each merge precision plus a 1 GiB copy of known size "
                          "(for hardware-counter runs), then exit")
     return ap.parse_args()
 
@@ -209,9 +216,51 @@ def bench_atrg3d(args):
     return 0
 
 
+def serve_leg(args):
+    """The request stream through serve.Server: Poisson arrivals (rate 50/s, seed 2), model uniform
+    over {2D Ising HOTRG, 3D Ising ATRG, 3D q=3 Potts ATRG}, chi uniform over --serve-chis, beta
+    uniform within 10% of the model's critical coupling, steps uniform in --serve-steps.  2D flows
+    run float64 for 10 steps then float32; ATRG flows run float64 (rSVD k = 2 chi, q = 2)."""
+    import torch
+
+    from trg_backup_amd import serve
+    chis = tuple(int(c) for c in args.serve_chis.split(","))
+    lo, hi = (int(c) for c in args.serve_steps.split(","))
+    reqs = serve.poisson_stream(args.serve_n, 50.0, 2, chis=chis, steps=(lo, hi))
+    srv = serve.Server()
+    t1 = time.perf_counter()
+    done = srv.run(reqs, realtime=True)
+    ts = time.perf_counter() - t1
+    summ = serve.summarize(done, ts)
+    twod = [f.req for f in done if f.req.model == "ising2d"]
+    ref = {f.req.rid: f.result for f in serve.Server(fp64_steps=10 ** 9).run(twod, realtime=False)} if twod else {}
+    err = max([abs(f.result - ref[f.req.rid]) for f in done if f.req.rid in ref] or [0.0])
+    by = {}
+    for f in done:
+        by.setdefault(f.req.model, []).append(f.finished_at - f.req.arrival)
+    torch.cuda.empty_cache()
+    return dict(summ, worst_lnZ_err_vs_fp64_2d=err, rejected=len(srv.rejected), wall_s=ts,
+                max_hbm_admitted_gb=srv.max_used / 2 ** 30,
+                latency_by_model={m: {"n": len(v), "p50_s": float(sorted(v)[len(v) // 2]), "max_s": max(v)}
+                                  for m, v in by.items()},
+                stream=f"{args.serve_n} Poisson requests (rate 50/s, seed 2), models {{2D Ising HOTRG, 3D Ising "
+                       f"ATRG, 3D Potts ATRG}}, chi in {set(chis)}, {lo}-{hi} RG steps, beta within 10% of beta_c")
+
+
+def bench_serve(args):
+    from trg_backup_amd import _lib
+    _lib.load()
+    res = serve_leg(args)
+    cfg = {"workload": res["stream"], "parallelism": "single GPU, continuous batching"}
+    extra = {"serving": res, "p50_latency_s": res["p50_latency_s"], "p99_latency_s": res["p99_latency_s"]}
+    out = _line(cfg, res["rg_steps"] / res["wall_s"], res["rg_steps"], 0, res["wall_s"], "f32/f64", None, extra)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def traffic_probe(args):
-    """One steady float32 step and one bf16-merge step of the bench batch, then a plain
-    1 GiB float32 copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    """One steady step of the bench batch with each merge precision (float32, bf16x3, bf16),
+    then a plain 1 GiB float32 copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
     import torch
 
     from trg_backup_amd import _lib, autotune, hotrg, models, ops
@@ -229,11 +278,12 @@ def traffic_probe(args):
         d = "X" if d == "Y" else "Y"
     T = ops.cast(T, torch.float32)
     autotune.enable(not args.no_tune)
-    for prec in (hotrg.FP32, hotrg.Precision(torch.float32, "f32", "bf16")):
-        # tune first (untimed), then the counted step
+    for prec in (hotrg.FP32, hotrg.Precision(torch.float32, "f32", "bf16x3"),
+                 hotrg.Precision(torch.float32, "f32", "bf16")):
+        # tune first, then the step whose dispatches are read
         hotrg.step2d(T, chi, d, prec)
         torch.cuda.synchronize()
-        print(f"[probe] step {d} {prec} tiles {autotune.table()}", flush=True)
+        print(f"[probe] counted step {d} merge={prec.merge}", flush=True)
         T = hotrg.step2d(T, chi, d, prec)
         torch.cuda.synchronize()
     autotune.enable(False)
@@ -267,6 +317,8 @@ def main():
         return bench_hotrg3d(args)
     if wl == "atrg3d":
         return bench_atrg3d(args)
+    if wl == "serve":
+        return bench_serve(args)
     import numpy as np
     import torch
 
@@ -276,8 +328,9 @@ def main():
     dev = torch.device("cuda:0")
     chi, nb, K = args.chi, args.nbeta, args.steps
     betas = [0.40 + 0.08 * i / (nb - 1) for i in range(nb)] if nb > 1 else [0.44]
-    policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32,
-                          bf16_merge_from=None if args.bf16_merge_from < 0 else args.bf16_merge_from)
+    opt = lambda v: None if v < 0 else v
+    policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32, bf16_env_from=opt(args.bf16_env_from),
+                          split_merge_from=opt(args.split_merge_from), bf16_merge_from=opt(args.bf16_merge_from))
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     T0 = torch.stack([models.initial_tensor("ising2d", b, dev) for b in betas])
 
@@ -326,7 +379,8 @@ def main():
     torch.cuda.empty_cache()
 
     # merge GEMM launches grouped by (storage, compute)
-    names = {(_lib.F32, _lib.F32): "f32", (_lib.F32, _lib.BF16): "bf16", (_lib.F64, _lib.F64): "f64"}
+    names = {(_lib.F32, _lib.F32): "f32", (_lib.F32, _lib.BF16): "bf16", (_lib.F32, _lib.BF16X3): "bf16x3",
+             (_lib.F64, _lib.F64): "f64"}
     groups = {}
     for (e0, e1, fl), cfg in zip(rec.events, rec.event_configs):
         groups.setdefault(names[cfg[:2]], []).append((e0.elapsed_time(e1), fl, cfg))
@@ -404,24 +458,11 @@ def main():
 
     # ---- serving: a Poisson stream of mixed solves (latency) ----
     if not args.no_serve:
-        from trg_backup_amd import serve
-        reqs = serve.poisson_stream(args.serve_n, 50.0, 2, chis=(16, 24, 32), steps=(20, 30))
-        srv = serve.Server(streams=2)
-        t1 = time.perf_counter()
-        done = srv.run(reqs, realtime=True)
-        ts = time.perf_counter() - t1
-        summ = serve.summarize(done, ts)
-        twod = [f.req for f in done if f.req.model == "ising2d"]
-        ref = {f.req.rid: f.result for f in serve.Server(fp64_steps=10 ** 9).run(twod, realtime=False)} if twod else {}
-        err = max([abs(f.result - ref[f.req.rid]) for f in done if f.req.rid in ref] or [0.0])
-        out["serving"] = dict(summ, worst_lnZ_err_vs_fp64_2d=err, rejected=len(srv.rejected),
-                              stream=f"{args.serve_n} Poisson requests (rate 50/s, seed 2), models "
-                                     "{2D Ising HOTRG, 3D Ising ATRG, 3D Potts ATRG}, chi in {16,24,32}, "
-                                     "20-30 RG steps, beta within 10% of beta_c")
-        out["p50_latency_s"] = summ["p50_latency_s"]
-        out["p99_latency_s"] = summ["p99_latency_s"]
-        log(f"serving: {summ}")
-        torch.cuda.empty_cache()
+        res = serve_leg(args)
+        out["serving"] = res
+        out["p50_latency_s"] = res["p50_latency_s"]
+        out["p99_latency_s"] = res["p99_latency_s"]
+        log(f"serving: {res}")
 
     # ---- CPU baseline: the NumPy float64 oracle on the host ----
     if not args.no_cpu:

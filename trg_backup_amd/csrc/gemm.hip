@@ -429,7 +429,9 @@ gemm_bf16_kernel(GemmArgs g) {
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-template <int BM, int BN, int BK, int WM, int WN, bool SPLIT>
+// DB: two LDS stages; the next slice is written into the stage the waves are
+// not reading, so one barrier per K slice suffices instead of two.
+template <int BM, int BN, int BK, int WM, int WN, bool SPLIT, bool DB>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
 gemm_bf16k_kernel(GemmArgs g) {
     constexpr int WAVES_M = BM / WM;
@@ -444,8 +446,10 @@ gemm_bf16k_kernel(GemmArgs g) {
     static_assert(AV * NT == BM * KQ && BV * NT == BN * KQ, "tile not divisible by threads");
     static_assert(BK % 16 == 0, "BK must be a multiple of 16");
 
-    __shared__ __attribute__((aligned(16))) __bf16 As[NS * BM * KP];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[NS * BN * KP];
+    constexpr int NB = DB ? 2 : 1;
+    constexpr int ASZ = NS * BM * KP, BSZ = NS * BN * KP;   // one stage
+    __shared__ __attribute__((aligned(16))) __bf16 As[NB * ASZ];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[NB * BSZ];
 
     int tm, tn;
     tile_coords(g, tm, tn);
@@ -496,30 +500,38 @@ gemm_bf16k_kernel(GemmArgs g) {
         *reinterpret_cast<bf16x4 *>(&S[off]) = hi;
         if constexpr (SPLIT) *reinterpret_cast<bf16x4 *>(&S[rows * KP + off]) = lo;
     };
-    auto store = [&]() {
+    auto store = [&](int stage) {
 #pragma unroll
         for (int i = 0; i < AV; ++i) {
             const int e = tid + i * NT;
-            put(As, BM, (e / KQ) * KP + 4 * (e % KQ), ra[i]);
+            put(As + stage * ASZ, BM, (e / KQ) * KP + 4 * (e % KQ), ra[i]);
         }
 #pragma unroll
         for (int i = 0; i < BV; ++i) {
             const int e = tid + i * NT;
-            put(Bs, BN, (e / KQ) * KP + 4 * (e % KQ), rb[i]);
+            put(Bs + stage * BSZ, BN, (e / KQ) * KP + 4 * (e % KQ), rb[i]);
         }
     };
 
     load(0);
-    store();
+    store(0);
     __syncthreads();
+    int cur = 0;
     for (int k0 = 0; k0 < g.K; k0 += BK) {
         const bool more = k0 + BK < g.K;
         if (more) load(k0 + BK);
-        bf16_slice<BM, BN, BK, KP, WM, WN, SPLIT>(As, Bs, acc, wm, wn, lane);
-        __syncthreads();
-        if (more) {
-            store();
+        bf16_slice<BM, BN, BK, KP, WM, WN, SPLIT>(As + cur * ASZ, Bs + cur * BSZ, acc, wm, wn, lane);
+        if constexpr (DB) {
+            // the other stage was last read before the previous barrier
+            if (more) store(cur ^ 1);
             __syncthreads();
+            cur ^= 1;
+        } else {
+            __syncthreads();
+            if (more) {
+                store(0);
+                __syncthreads();
+            }
         }
     }
 
@@ -586,8 +598,9 @@ static bool kvec_ok(const GemmArgs &g, bool amc, bool bnc) {
 // KONLY: the tile exists for the K-contiguous kernel only (its generic
 // counterpart would spill registers); other layouts run tile 256x128x32 w64x64
 // (128x128x32 w64x64 when SPLIT), which gives the same bits.
-template <int BM, int BN, int BK, int WM, int WN, bool SPLIT, bool KONLY = false>
+template <int BM, int BN, int BK, int WM, int WN, bool SPLIT, bool KONLY = false, bool DB = false>
 static int launch_bf16(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
+    static_assert(!DB || KONLY, "double-buffered tiles exist for the K-contiguous kernel only");
     if constexpr (KONLY) {
         if (!kvec_ok(g, amc, bnc)) {
             if constexpr (SPLIT) return launch_bf16<128, 128, 32, 64, 64, true>(g, batch, amc, bnc, s);
@@ -599,7 +612,7 @@ static int launch_bf16(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s)
     dim3 grid(g.tiles_m * g.tiles_n, batch);
     dim3 block((BM / WM) * (BN / WN) * 64);
     if (KONLY || kvec_ok(g, amc, bnc)) {
-        hipLaunchKernelGGL((gemm_bf16k_kernel<BM, BN, BK, WM, WN, SPLIT>), grid, block, 0, s, g);
+        hipLaunchKernelGGL((gemm_bf16k_kernel<BM, BN, BK, WM, WN, SPLIT, DB>), grid, block, 0, s, g);
         return check_launch();
     }
     if constexpr (!KONLY) {
@@ -653,6 +666,8 @@ static int gemm_dispatch(int dtype, int compute, int config, GemmArgs g, int bat
             case 7: return launch_bf16<256, 256, 32, 128, 64, false, true>(g, batch, amc, bnc, s);
             case 8: return launch_bf16<256, 256, 16, 128, 64, false>(g, batch, amc, bnc, s);
             case 9: return launch_bf16<128, 256, 32, 64, 64, false>(g, batch, amc, bnc, s);
+            case 10: return launch_bf16<256, 256, 32, 128, 64, false, true, true>(g, batch, amc, bnc, s);
+            case 11: return launch_bf16<256, 128, 32, 64, 64, false, true, true>(g, batch, amc, bnc, s);
             default: return TRG_BAD_ARG;
         }
     }
@@ -665,6 +680,8 @@ static int gemm_dispatch(int dtype, int compute, int config, GemmArgs g, int bat
             case 3: return launch_bf16<128, 256, 32, 64, 64, true>(g, batch, amc, bnc, s);
             case 4: return launch_bf16<256, 256, 32, 128, 64, true, true>(g, batch, amc, bnc, s);
             case 5: return launch_bf16<256, 128, 32, 128, 64, true, true>(g, batch, amc, bnc, s);
+            case 6: return launch_bf16<256, 128, 32, 64, 64, true, true, true>(g, batch, amc, bnc, s);
+            case 7: return launch_bf16<128, 256, 32, 64, 64, true, true, true>(g, batch, amc, bnc, s);
             default: return TRG_BAD_ARG;
         }
     }
@@ -710,8 +727,8 @@ extern "C" int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, 
 
 extern "C" int trg_gemm_config_count(int dtype, int compute) {
     if (dtype == TRG_F32 && compute == TRG_F32) return 8;
-    if (dtype == TRG_F32 && compute == TRG_BF16) return 10;
-    if (dtype == TRG_F32 && compute == TRG_BF16X3) return 6;
+    if (dtype == TRG_F32 && compute == TRG_BF16) return 12;
+    if (dtype == TRG_F32 && compute == TRG_BF16X3) return 8;
     if (dtype == TRG_F64 && compute == TRG_F64) return 4;
     return 0;
 }
@@ -721,9 +738,11 @@ extern "C" const char *trg_gemm_config_name(int dtype, int compute, int config) 
                                 "64x256x16 w64x64", "256x128x32 w64x64", "128x256x32 w64x64", "256x128x16 w128x64"};
     static const char *bf[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "256x128x32 w128x64",
                                "128x256x32 w64x128", "128x128x64 w64x64", "256x128x64 w128x64",
-                               "256x256x32 w128x64", "256x256x16 w128x64", "128x256x32 w64x64"};
+                               "256x256x32 w128x64", "256x256x16 w128x64", "128x256x32 w64x64",
+                               "256x256x32 w128x64 db", "256x128x32 w64x64 db"};
     static const char *bx3[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "128x256x32 w64x64",
-                                "256x256x32 w128x64", "256x128x32 w128x64"};
+                                "256x256x32 w128x64", "256x128x32 w128x64", "256x128x32 w64x64 db",
+                                "128x256x32 w64x64 db"};
     static const char *f64[] = {"128x128x16 w64x64", "64x64x16 w32x32", "128x64x16 w64x32", "64x128x16 w32x64"};
     if (config < 0 || config >= trg_gemm_config_count(dtype, compute)) return "";
     if (dtype == TRG_F32 && compute == TRG_F32) return f32[config];
