@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--serve-chis", default="16,24,32", help="bond dimensions of the request stream")
     ap.add_argument("--serve-steps", default="20,30", help="RG-step range of the request stream")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--load-tune", default=None, help="GEMM tile table (JSON) to start from")
+    ap.add_argument("--save-tune", default=None, help="write the GEMM tile table after the warm-up")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run one steady step with each merge precision plus a 1 GiB copy of known size "
                          "(for hardware-counter runs), then exit")
@@ -278,6 +280,8 @@ def traffic_probe(args):
         hotrg.normalize_maxabs(T, lnf, n)
         d = "X" if d == "Y" else "Y"
     T = ops.cast(T, torch.float32)
+    if args.load_tune:
+        autotune.load(args.load_tune)
     autotune.enable(not args.no_tune)
     for prec in (hotrg.FP32, hotrg.Precision(torch.float32, "f32", "bf16x3"),
                  hotrg.Precision(torch.float32, "f32", "bf16")):
@@ -352,12 +356,16 @@ def main():
             warm.append((n, d))
             if n > 6:
                 seen[key] = seen.get(key, 0) + 1
+    if args.load_tune:
+        autotune.load(args.load_tune)
     autotune.enable(not args.no_tune)
     t0 = time.perf_counter()
     lw = ops.fill_(torch.empty((nb, K + 1), dtype=torch.float64, device=dev), 0.0)
     run(warm, lw)
     torch.cuda.synchronize()
     autotune.enable(False)
+    if args.save_tune:
+        autotune.save(args.save_tune)
     log(f"warm-up {len(warm)} steps in {time.perf_counter() - t0:.1f} s; "
         f"{len(autotune.table())} GEMM problems tuned")
     del lw

@@ -143,7 +143,13 @@ def test_hotrg2d_bf16_merge_close_to_fp32(gpu):
 
 def test_hotrg3d_batched_equals_alone(gpu):
     """Couplings from default_rng(0).uniform(0.20, 0.24): a batched 3D solve gives each
-    instance exactly its single-instance result (float32 steps after the first three)."""
+    instance exactly its single-instance result (float32 steps after the first three), the
+    mixed schedule stays within 1e-8 of float64, and the float64 batch matches the oracle.
+
+    The oracle comparison runs at chi = 6: at chi = 8 the step-2 environment has a four-fold
+    degenerate eigenvalue straddling the cut (positions 7-10), where the reference's
+    eigh(...)[:, D-Dcut:] keeps an implementation-defined half of the multiplet, so two
+    correct eigensolvers legitimately differ there (by 7e-6 in ln Z/V)."""
     from trg_backup_amd import hotrg
     betas = np.random.default_rng(0).uniform(0.20, 0.24, 128)[:3]
     T0 = torch.stack([torch.as_tensor(models.ising3d_tensor(b)) for b in betas]).to(gpu)
@@ -154,6 +160,12 @@ def test_hotrg3d_batched_equals_alone(gpu):
         Ti, li = hotrg.renorm3d(T0[i:i + 1], 8, 2, 2, 2, pol)
         zi = hotrg.lnz3d(Ti, li, 6).cpu().numpy()
         assert zi[0] == zb[i], (i, zi[0], zb[i])
-    ref = o3.ln_Z_over_V(models.ising3d_tensor(float(betas[0])), 8, 2, 2, 2)
-    print(f"\n[hotrg3d batched chi=8] {zb} oracle[0]={ref!r} diff={zb[0] - ref:.3e}", flush=True)
-    assert abs(zb[0] - ref) < 1e-7
+    Td, ld = hotrg.renorm3d(T0, 8, 2, 2, 2, hotrg.ALL_FP64)
+    zd = hotrg.lnz3d(Td, ld, 6).cpu().numpy()
+    T6, l6 = hotrg.renorm3d(T0, 6, 2, 2, 2, hotrg.ALL_FP64)
+    z6 = hotrg.lnz3d(T6, l6, 6).cpu().numpy()
+    ref = np.array([o3.ln_Z_over_V(models.ising3d_tensor(float(b)), 6, 2, 2, 2) for b in betas])
+    print(f"\n[hotrg3d batched] chi=8 mixed-fp64 {np.abs(zb - zd)}; chi=6 fp64-oracle {np.abs(z6 - ref)}",
+          flush=True)
+    assert np.all(np.abs(zb - zd) < 1e-8)
+    assert np.all(np.abs(z6 - ref) < 1e-12)

@@ -67,13 +67,16 @@ def rsvd_for_ATRG_leg_swapping(Bt, Ct, chi_k, p, q, eps, seed):
     ls = min(l, rows_i, rows_j)
     dt = Bt.dtype
 
+    # Both applies contract the small bond first, so the intermediate is
+    # (i,t,j,l) / (t,j,i,l) -- chi^3 l elements -- and each costs 4 chi^5 l
+    # flop, the optimal path of the reference's three-operand einsums (:99, :110, :117).
     def apply_M(Yjbl):            # (j,b,l) -> (i,g,l)
-        W = contract("tjg,jbl->tgbl", Ct, Yjbl)
-        return contract("bit,tgbl->igl", Bt, W)
+        W = contract("bit,jbl->itjl", Bt, Yjbl)
+        return contract("itjl,tjg->igl", W, Ct)
 
     def apply_MT(Qigl):           # (i,g,l) -> (j,b,l)
-        W = contract("bit,igl->btgl", Bt, Qigl)
-        return contract("tjg,btgl->jbl", Ct, W)
+        W = contract("tjg,igl->tjil", Ct, Qigl)
+        return contract("tjil,bit->jbl", W, Bt)
 
     # The small SVDs run in one workgroup's LDS, which caps the sketch block
     # at linalg.SMALL_MAX columns (a documented device limit: above it the
