@@ -58,17 +58,21 @@ int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, double alph
 /*
  * trg_gemm with an explicit tile configuration (0 <= config <
  * trg_gemm_config_count(dtype, compute); -1 = the shape heuristic of
- * trg_gemm).  All configurations of one (dtype, compute) pair issue the same
- * MFMA instruction over K in the same order, so results are bitwise equal
- * across configurations; only speed differs.  Used by the autotuner that
- * stands in for cuTENSOR's plan selection behind the reference's
- * cuquantum.contract calls (trg/HOTRG_2d.py:26-27, trg/HOTRG_3d.py:42-45).
+ * trg_gemm) and an optional two-level output index: with cdm > 0 row m of
+ * C[b] sits at (m / cdm) * csm + (m % cdm) * rsC, with cdn > 0 column n at
+ * (n / cdn) * csn + (n % cdn) * csC (0 = the plain strides), so the product
+ * lands directly in a permuted tensor layout.  All configurations of one
+ * (dtype, compute) pair issue the same MFMA instruction over K in the same
+ * order, so results are bitwise equal across configurations; only speed
+ * differs.  Used by the autotuner that stands in for cuTENSOR's plan
+ * selection behind the reference's cuquantum.contract calls
+ * (trg/HOTRG_2d.py:26-27,65-67, trg/HOTRG_3d.py:42-45).
  */
 int trg_gemm_ex(int config, int dtype, int compute, int batch, int M, int N, int K, double alpha,
                 const void *A, long long sAb, long long rsA, long long csA,
                 const void *B, long long sBb, long long rsB, long long csB,
                 double beta, void *C, long long sCb, long long rsC, long long csC,
-                void *stream);
+                int cdm, long long csm, int cdn, long long csn, void *stream);
 int trg_gemm_config_count(int dtype, int compute);
 const char *trg_gemm_config_name(int dtype, int compute, int config);
 

@@ -189,7 +189,7 @@ def test_gemm_tile_configs_bitwise_equal(gpu, dtype, compute, shape, ta, tb):
         C = torch.empty_like(C0)
         _lib.call("trg_gemm_ex", cfg, dt, cc, 3, M, N, K, 1.0, Av.data_ptr(), Av.stride(0), Av.stride(1),
                   Av.stride(2), Bv.data_ptr(), Bv.stride(0), Bv.stride(1), Bv.stride(2), 0.0, C.data_ptr(),
-                  M * N, N, 1, ops.stream_ptr())
+                  M * N, N, 1, 0, 0, 0, 0, ops.stream_ptr())
         assert torch.equal(C, C0), (cfg, autotune.config_name(dt, cc, cfg))
 
 
@@ -232,3 +232,23 @@ def test_bf16x3_accuracy_between_bf16_and_fp32(gpu):
          for c in (None, "bf16x3", "bf16")}
     print(f"\n[bf16x3] rel err fp32 {e[None]:.2e} bf16x3 {e['bf16x3']:.2e} bf16 {e['bf16']:.2e}", flush=True)
     assert e["bf16x3"] < 5e-5 and e["bf16x3"] < e["bf16"] / 50 and e[None] < e["bf16x3"]
+
+
+@pytest.mark.parametrize("compute", [None, "bf16", "bf16x3"])
+def test_gemm_two_level_output_index(gpu, compute):
+    """A product written through a two-level output index equals the plain product
+    permuted into the same layout, bit for bit (same kernel, only the epilogue
+    addresses differ); beta accumulation reads through the same index."""
+    from trg_backup_amd import ops
+    rng = np.random.default_rng(21)
+    B, M1, M2, N1, N2, K = 3, 6, 16, 8, 12, 40          # rows (M1 | M2), columns (N1 | N2)
+    A = _dev(rng.standard_normal((B, M1 * M2, K)), gpu).float()
+    Bm = _dev(rng.standard_normal((B, K, N1 * N2)), gpu).float()
+    plain = ops.gemm(A, Bm, compute=compute)                           # [(m1,m2),(n1,n2)]
+    want = ops.contiguous(plain.view(B, M1, M2, N1, N2).permute(0, 2, 3, 1, 4))   # [m2,n1,m1,n2]
+    out = torch.empty(B * M2 * N1 * M1 * N2, dtype=torch.float32, device=gpu)
+    lay = ([(M1, N2), (M2, N1 * M1 * N2)], [(N1, M1 * N2), (N2, 1)])
+    ops.gemm(A, Bm, out, compute=compute, out_layout=lay)
+    assert torch.equal(out.view(B, M2, N1, M1, N2), want)
+    ops.gemm(A, Bm, out, beta=1.0, alpha=-1.0, compute=compute, out_layout=lay)
+    assert float(out.abs().max()) == 0.0

@@ -1,0 +1,46 @@
+"""CPU checks of host-side layout logic: the merge GEMM chain writes its products through
+two-level output indices (ops.gemm's out_layout); emulating that GEMM on the CPU, the chain
+must equal the einsum acke,bdel,abi,cdj->ijkl of trg/HOTRG_2d.py:65 for both operand layouts
+(K-outer for fp32 MFMA, K-contiguous for bf16 MFMA) and with the c leg split into chunks."""
+import pytest
+import torch
+
+from trg_backup_amd import hotrg
+
+
+def _cpu_gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_layout=None):
+    P = alpha * torch.matmul(A.double(), B.double())
+    if out_layout is None:
+        if C is None:
+            return P.to(A.dtype)
+        C.copy_((P + beta * C.double()).to(C.dtype))
+        return C
+    batch, M, N = P.shape
+
+    def offsets(dims, n):
+        idx = torch.arange(n)
+        if len(dims) == 1:
+            return idx * dims[0][1]
+        return (idx // dims[1][0]) * dims[0][1] + (idx % dims[1][0]) * dims[1][1]
+
+    addr = (offsets(out_layout[0], M)[:, None] + offsets(out_layout[1], N)[None, :]).flatten()
+    assert sorted(addr.tolist()) == list(range(M * N)), "output layout is not a bijection"
+    flat = C.view(batch, M * N)
+    for b in range(batch):
+        flat[b, addr] = P[b].flatten().to(C.dtype)
+    return None
+
+
+@pytest.mark.parametrize("shape", [(2, 4, 3, 5), (1, 6, 6, 6), (3, 5, 2, 4)])
+@pytest.mark.parametrize("compute", [None, "bf16"])
+def test_merge2d_chain_layouts(monkeypatch, shape, compute):
+    B, D, Dy, chi = shape
+    monkeypatch.setattr(hotrg, "gemm", _cpu_gemm)
+    monkeypatch.setattr(hotrg, "permute", lambda t, perm: t.permute(*perm).contiguous())
+    monkeypatch.setattr(hotrg, "INSTANCE_BUDGET", 2 * Dy * Dy * D * max(chi, Dy) * 8 * 2)   # 2 c per chunk
+    g = torch.Generator().manual_seed(sum(shape))
+    T = torch.randn(B, D, D, Dy, Dy, dtype=torch.float64, generator=g)
+    U = torch.randn(B, D, D, chi, dtype=torch.float64, generator=g)
+    ref = torch.einsum("zacke,zbdel,zabi,zcdj->zijkl", T, T, U, U)
+    out = hotrg.merge2d(T, T, U, U, compute=compute)
+    assert (out - ref).abs().max().item() < 1e-10

@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtrg.so")
 
 F32, F64, BF16, I32, BF16X3 = 0, 1, 2, 3, 4
+ABI = 2          # must match the "abi N" revision in trg_version()
 
 c_int, c_ll, c_dbl, c_vp, c_ull, c_uint = (ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p,
                                            ctypes.c_ulonglong, ctypes.c_uint)
@@ -27,7 +28,7 @@ SIGNATURES = {
                  c_dbl, c_vp, c_ll, c_ll, c_ll, c_vp],
     "trg_gemm_ex": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_dbl,
                     c_vp, c_ll, c_ll, c_ll, c_vp, c_ll, c_ll, c_ll,
-                    c_dbl, c_vp, c_ll, c_ll, c_ll, c_vp],
+                    c_dbl, c_vp, c_ll, c_ll, c_ll, c_int, c_ll, c_int, c_ll, c_vp],
     "trg_gemm_config_count": [c_int, c_int],
     "trg_gemm_config_name": [c_int, c_int, c_int],
     "trg_copy_strided": [c_int, c_int, c_int, P_LL, c_vp, P_LL, c_vp, P_LL, c_vp],
@@ -72,6 +73,10 @@ def load():
         raise TrgError(f"{LIB_PATH} is not built; run `python -m trg_backup_amd.build` "
                        "(or __graft_entry__.build()) first")
     lib = ctypes.CDLL(LIB_PATH)
+    lib.trg_version.restype = ctypes.c_char_p
+    got = lib.trg_version().decode()
+    if f" abi {ABI} " not in got:
+        raise TrgError(f"{LIB_PATH} reports {got!r}, this binding needs abi {ABI}; rebuild the library")
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = args

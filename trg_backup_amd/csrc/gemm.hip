@@ -33,7 +33,20 @@ struct GemmArgs {
     double alpha, beta;
     int tiles_n;   // number of BN tiles along N
     int tiles_m;   // number of BM tiles along M
+    // Two-level output index (0 = plain): row m of C sits at
+    // (m / cdm) * csm + (m % cdm) * rsC, column n at (n / cdn) * csn + (n % cdn) * csC,
+    // so a GEMM can write straight into a permuted tensor layout.
+    int cdm, cdn;
+    long long csm, csn;
 };
+
+__device__ inline long long c_offset(const GemmArgs &g, int m, int n) {
+    const long long om = g.cdm ? (long long)(m / g.cdm) * g.csm + (long long)(m % g.cdm) * g.rsC
+                               : (long long)m * g.rsC;
+    const long long on = g.cdn ? (long long)(n / g.cdn) * g.csn + (long long)(n % g.cdn) * g.csC
+                               : (long long)n * g.csC;
+    return om + on;
+}
 
 // ---------------------------------------------------------------------------
 // MFMA traits: operand lane map and accumulator lane map.
@@ -245,7 +258,7 @@ gemm_kernel(GemmArgs g) {
                 const int m = m0 + wm * WM + i * MF::MB + MF::c_row(lane, r);
                 const int n = n0 + wn * WN + j * MF::MB + MF::c_col(lane);
                 if (m < g.M && n < g.N) {
-                    const long long idx = (long long)m * g.rsC + (long long)n * g.csC;
+                    const long long idx = c_offset(g, m, n);
                     T v = alpha * acc[i][j][r];
                     if (g.beta != 0.0) v += beta * C[idx];
                     C[idx] = v;
@@ -407,7 +420,7 @@ gemm_bf16_kernel(GemmArgs g) {
                 const int m = m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                 const int n = n0 + wn * WN + j * 32 + (lane & 31);
                 if (m < g.M && n < g.N) {
-                    const long long idx = (long long)m * g.rsC + (long long)n * g.csC;
+                    const long long idx = c_offset(g, m, n);
                     float v = alpha * acc[i][j][r];
                     if (g.beta != 0.0) v += beta * C[idx];
                     C[idx] = v;
@@ -545,7 +558,7 @@ gemm_bf16k_kernel(GemmArgs g) {
                 const int m = m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                 const int n = n0 + wn * WN + j * 32 + (lane & 31);
                 if (m < g.M && n < g.N) {
-                    const long long idx = (long long)m * g.rsC + (long long)n * g.csC;
+                    const long long idx = c_offset(g, m, n);
                     float v = alpha * acc[i][j][r];
                     if (g.beta != 0.0) v += beta * C[idx];
                     C[idx] = v;
@@ -755,8 +768,8 @@ extern "C" int trg_gemm_ex(int config, int dtype, int compute, int batch, int M,
                            const void *A, long long sAb, long long rsA, long long csA,
                            const void *B, long long sBb, long long rsB, long long csB,
                            double beta, void *C, long long sCb, long long rsC, long long csC,
-                           void *stream) {
-    if (batch < 0 || M < 0 || N < 0 || K < 0) return TRG_BAD_ARG;
+                           int cdm, long long csm, int cdn, long long csn, void *stream) {
+    if (batch < 0 || M < 0 || N < 0 || K < 0 || cdm < 0 || cdn < 0) return TRG_BAD_ARG;
     if (batch == 0 || M == 0 || N == 0) return TRG_OK;
     if (!A || !B || !C) return TRG_BAD_ARG;
     if (batch > 65535) return TRG_SHAPE;
@@ -765,6 +778,7 @@ extern "C" int trg_gemm_ex(int config, int dtype, int compute, int batch, int M,
     const bool bnc = (csB == 1);
     if (!amc && csA != 1) return TRG_BAD_ARG;
     if (!bnc && rsB != 1) return TRG_BAD_ARG;
-    GemmArgs g{M, N, K, A, sAb, rsA, csA, B, sBb, rsB, csB, C, sCb, rsC, csC, alpha, beta, 0, 0};
+    GemmArgs g{M, N, K, A, sAb, rsA, csA, B, sBb, rsB, csB, C, sCb, rsC, csC, alpha, beta, 0, 0,
+               cdm, cdn, csm, csn};
     return gemm_dispatch(dtype, compute, config, g, batch, amc, bnc, as_stream(stream));
 }

@@ -170,7 +170,7 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
     """acke,bdel,abi,cdj->ijkl (trg/HOTRG_2d.py:65/67) for batched operands.
 
     Z1[c,k,e,b,i] = sum_a T1[a,(c,k,e)] Ua[a,(b,i)]          2 Dx^3 Dy^2 chi
-    Z2[(c,k,i),(d,l)] = sum_(e,b) Z1[e,b,(c,k,i)] T2[(e,b),(d,l)]   2 Dx^3 Dy^3 chi
+    Z2[(c,k,i),(l,d)] = sum_(e,b) Z1[e,b,(c,k,i)] T2[(e,b),(l,d)]   2 Dx^3 Dy^3 chi
     out[(k,i,l),j] = sum_(c,d) Z2[k,i,l,(c,d)] Ub[(c,d),j]      2 Dx^2 Dy^2 chi^2
     The c leg is processed in chunks that fit MERGE_BUDGET; the last GEMM
     accumulates over the chunks.
@@ -192,14 +192,19 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
             ops.copy_into(res[b0:b0 + outs[i].shape[0]], outs[i])
         return res
     # fp32 MFMA wants K-outer operands, bf16 MFMA K-contiguous ones (8
-    # consecutive k per lane); the permutations below produce either layout.
+    # consecutive k per lane).  The first two GEMMs write their products
+    # straight into the layout the next GEMM reads (two-level output index),
+    # so no chi^5 intermediate is permuted; the column order of each product
+    # keeps its innermost index contiguous in the destination (coalesced
+    # epilogue stores).
     kinner = compute in ("bf16", "bf16x3")
     if kinner:
-        Tp = permute(T2, (0, 2, 4, 3, 1)).view(B, Dx * Dy, Dy * Dx).transpose(1, 2)   # [(e,b),(d,l)]
+        Tp = permute(T2, (0, 4, 2, 3, 1)).view(B, Dy * Dx, Dy * Dx).transpose(1, 2)   # [(e,b),(l,d)]
+        Uam = permute(Ua, (0, 1, 3, 2)).view(B, Dx, chi_a * Dx)                     # [a,(i,b)]
     else:
-        Tp = permute(T2, (0, 3, 1, 2, 4)).view(B, Dy * Dx, Dx * Dy)      # [(e,b),(d,l)]
+        Tp = permute(T2, (0, 3, 1, 4, 2)).view(B, Dy * Dx, Dy * Dx)      # [(e,b),(l,d)]
+        Uam = Ua.reshape(B, Dx, Dx * chi_a)                               # [a,(b,i)]
     T1m = T1.reshape(B, Dx, Dx * Dy * Dy)                                 # [a,(c,k,e)]
-    Uam = Ua.reshape(B, Dx, Dx * chi_a)                                   # [a,(b,i)]
     Ubm = Ub.reshape(B, Dx * Dx, chi_b)                                   # [(c,d),j]
     out = torch.empty((B, Dy, chi_a, Dy, chi_b), dtype=T1.dtype, device=T1.device)  # [k,i,l,j]
     outm = out.view(B, Dy * chi_a * Dy, chi_b)
@@ -207,18 +212,24 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
         c1 = min(Dx, c0 + cc)
         nc = c1 - c0
         A = T1m[:, :, c0 * Dy * Dy:c1 * Dy * Dy].transpose(1, 2)          # (B, (c,k,e), a)
-        Z1 = gemm(A, Uam, compute=compute)                                 # [(c,k,e),(b,i)]
+        Z1p = torch.empty(B * nc * Dy * Dy * Dx * chi_a, dtype=T1.dtype, device=T1.device)
         if kinner:
-            Z1p = permute(Z1.view(B, nc, Dy, Dy, Dx, chi_a), (0, 1, 2, 5, 3, 4))   # [c,k,i,e,b]
+            # [c,k,i,e,b]: rows (c,k | e), columns (i | b)
+            gemm(A, Uam, Z1p, compute=compute,
+                 out_layout=([(nc * Dy, chi_a * Dy * Dx), (Dy, Dx)], [(chi_a, Dy * Dx), (Dx, 1)]))
             Z2a = Z1p.view(B, nc * Dy * chi_a, Dy * Dx)
         else:
-            Z1p = permute(Z1.view(B, nc, Dy, Dy, Dx, chi_a), (0, 3, 4, 1, 2, 5))   # [e,b,c,k,i]
+            # [e,b,c,k,i]: rows (c,k | e), columns (b | i)
+            gemm(A, Uam, Z1p, compute=compute,
+                 out_layout=([(nc * Dy, chi_a), (Dy, Dx * nc * Dy * chi_a)], [(Dx, nc * Dy * chi_a), (chi_a, 1)]))
             Z2a = Z1p.view(B, Dy * Dx, nc * Dy * chi_a).transpose(1, 2)
-        Z2 = gemm(Z2a, Tp, compute=compute, tag="merge2d_z2")             # [(c,k,i),(d,l)]
-        Z2p = permute(Z2.view(B, nc, Dy, chi_a, Dx, Dy), (0, 2, 3, 5, 1, 4))   # [k,i,l,c,d]
+        # Z2 [(c,k,i),(l,d)] -> [k,i,l,c,d]: rows (c | k,i), columns (l | d)
+        Z2p = torch.empty(B * Dy * chi_a * Dy * nc * Dx, dtype=T1.dtype, device=T1.device)
+        gemm(Z2a, Tp, Z2p, compute=compute, tag="merge2d_z2",
+             out_layout=([(nc, Dx), (Dy * chi_a, Dy * nc * Dx)], [(Dy, nc * Dx), (Dx, 1)]))
         gemm(Z2p.view(B, Dy * chi_a * Dy, nc * Dx), Ubm[:, c0 * Dx:c1 * Dx, :], outm,
              beta=0.0 if c0 == 0 else 1.0, compute=compute)
-        del Z1, Z1p, Z2, Z2p
+        del Z1p, Z2p
     return permute(out, (0, 2, 4, 1, 3))                                   # [i,j,k,l]
 
 

@@ -3,6 +3,8 @@
 torch is used for device memory and the current stream only; every
 arithmetic operation below is a libtrg kernel.
 """
+import math
+
 import torch
 
 from . import _lib, autotune
@@ -129,24 +131,42 @@ class GemmRecorder:
 _RECORDERS = []
 
 
-def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None):
+def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_layout=None):
     """C[b] = alpha A[b] B[b] + beta C[b] for 2D or batched 3D strided views.
 
     A is (M, K) or (batch, M, K), B is (K, N) or (batch, K, N).  Views may
     be transposed as long as one of the two matrix strides of A and of B is 1.
     compute: None (same as storage), "bf16" / "bf16x3" (f32 storage only) or a dtype code.
+    out_layout: write the product into a permuted tensor instead of a matrix
+    view: C is then the destination tensor (contiguous, batch-major) and
+    out_layout = (rows, cols), each a list of (size, stride) from outer to
+    inner of at most two levels, with the row / column index of the product
+    running over them in that order.
     """
     require_device(A, B)
     batched = A.dim() == 3
     if not batched:
         A, B = A.unsqueeze(0), B.unsqueeze(0)
-        if C is not None:
+        if C is not None and out_layout is None:
             C = C.unsqueeze(0)
     bt, M, K = A.shape
     bt2, K2, N = B.shape
     if K != K2 or (bt != bt2 and bt2 != 1 and bt != 1):
         raise _lib.TrgError(f"gemm shape mismatch {tuple(A.shape)} x {tuple(B.shape)}")
     batch = max(bt, bt2)
+    split = (0, 0, 0, 0)
+    if out_layout is not None:
+        if C is None or not C.is_contiguous() or C.numel() != batch * M * N:
+            raise _lib.TrgError("out_layout needs a contiguous destination with batch*M*N elements")
+        (rows, cols), lv = out_layout, []
+        for dims, n in ((rows, M), (cols, N)):
+            if not 1 <= len(dims) <= 2 or math.prod(d for d, _ in dims) != n:
+                raise _lib.TrgError(f"out_layout {out_layout} does not factor the {M}x{N} product")
+            lv.append((0, 0, dims[0][1]) if len(dims) == 1 else (dims[1][0], dims[0][1], dims[1][1]))
+        (dm, sm, rsC), (dn, sn, csC) = lv
+        split = (dm, sm, dn, sn)
+        C = C.view(batch, M * N)
+        Cstr = (M * N if batch > 1 else 0, rsC, csC)
     if C is None:
         C = torch.empty((batch, M, N), dtype=A.dtype, device=A.device)
         beta = 0.0
@@ -185,24 +205,28 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None):
     if dt == F64:
         cc = F64
     opnds = (ptr(A), sA, rsA, csA, ptr(B), sB, rsB, csB)
+    if out_layout is None:
+        Cstr = (C.stride(0) if batch > 1 else 0, C.stride(1), C.stride(2))
     config = -1
     if autotune.active():
         def launch(cfg, out):
             call("trg_gemm_ex", cfg, dt, cc, batch, M, N, K, float(alpha), *opnds,
-                 0.0, ptr(out), M * N if batch > 1 else 0, N, 1, stream_ptr())
-        own = beta == 0.0 and C.is_contiguous() and C.shape == (batch, M, N)
+                 0.0, ptr(out), M * N if batch > 1 else 0, N, 1, 0, 0, 0, 0, stream_ptr())
+        own = out_layout is None and beta == 0.0 and C.is_contiguous() and C.shape == (batch, M, N)
         config = autotune.choose(dt, cc, batch, M, N, K, opnds, launch, C if own else None)
     recs = [r for r in _RECORDERS if tag is not None and r.tag == tag]
     if recs:
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record()
     call("trg_gemm_ex", config, dt, cc, batch, M, N, K, float(alpha), *opnds,
-         float(beta), ptr(C), C.stride(0) if batch > 1 else 0, C.stride(1), C.stride(2), stream_ptr())
+         float(beta), ptr(C), *Cstr, *split, stream_ptr())
     if recs:
         ev1.record()
         for r in recs:
             r.events.append((ev0, ev1, 2.0 * M * N * K * batch))
             r.event_configs.append((dt, cc, config, M, N, K, batch))
+    if out_layout is not None:
+        return None
     return C if batched else C[0]
 
 
