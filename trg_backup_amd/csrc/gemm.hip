@@ -40,12 +40,43 @@ struct GemmArgs {
     long long csm, csn;
 };
 
-__device__ inline long long c_offset(const GemmArgs &g, int m, int n) {
-    const long long om = g.cdm ? (long long)(m / g.cdm) * g.csm + (long long)(m % g.cdm) * g.rsC
-                               : (long long)m * g.rsC;
-    const long long on = g.cdn ? (long long)(n / g.cdn) * g.csn + (long long)(n % g.cdn) * g.csC
-                               : (long long)n * g.csC;
-    return om + on;
+__device__ inline long long c_row_offset(const GemmArgs &g, int m) {
+    return g.cdm ? (long long)(m / g.cdm) * g.csm + (long long)(m % g.cdm) * g.rsC : (long long)m * g.rsC;
+}
+__device__ inline long long c_col_offset(const GemmArgs &g, int n) {
+    return g.cdn ? (long long)(n / g.cdn) * g.csn + (long long)(n % g.cdn) * g.csC : (long long)n * g.csC;
+}
+
+// Epilogue C = alpha * acc + beta * C through the (possibly two-level) output
+// index.  Column offsets are computed once per column block and row offsets
+// once per accumulator row, so no index division sits in the inner loop.
+// row(i, r) / col(j) give the C row of accumulator register r of block i and
+// the C column of block j for this lane.
+template <typename T, int TM, int TN, int NACC, typename Acc, typename RowF, typename ColF>
+__device__ inline void store_c(const GemmArgs &g, T *__restrict__ C, Acc (&acc)[TM][TN], RowF row, ColF col) {
+    const T alpha = (T)g.alpha, beta = (T)g.beta;
+    long long on[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        const int n = col(j);
+        on[j] = n < g.N ? c_col_offset(g, n) : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int r = 0; r < NACC; ++r) {
+            const int m = row(i, r);
+            if (m >= g.M) continue;
+            const long long om = c_row_offset(g, m);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                if (on[j] < 0) continue;
+                const long long idx = om + on[j];
+                T v = alpha * acc[i][j][r];
+                if (g.beta != 0.0) v += beta * C[idx];
+                C[idx] = v;
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -248,22 +279,9 @@ gemm_kernel(GemmArgs g) {
         }
     }
 
-    const T alpha = (T)g.alpha, beta = (T)g.beta;
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int r = 0; r < MF::NACC; ++r) {
-                const int m = m0 + wm * WM + i * MF::MB + MF::c_row(lane, r);
-                const int n = n0 + wn * WN + j * MF::MB + MF::c_col(lane);
-                if (m < g.M && n < g.N) {
-                    const long long idx = c_offset(g, m, n);
-                    T v = alpha * acc[i][j][r];
-                    if (g.beta != 0.0) v += beta * C[idx];
-                    C[idx] = v;
-                }
-            }
+    store_c<T, TM, TN, MF::NACC>(
+        g, C, acc, [&](int i, int r) { return m0 + wm * WM + i * MF::MB + MF::c_row(lane, r); },
+        [&](int j) { return n0 + wn * WN + j * MF::MB + MF::c_col(lane); });
 }
 
 // ---------------------------------------------------------------------------
@@ -410,22 +428,9 @@ gemm_bf16_kernel(GemmArgs g) {
         }
     }
 
-    const float alpha = (float)g.alpha, beta = (float)g.beta;
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int m = m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                const int n = n0 + wn * WN + j * 32 + (lane & 31);
-                if (m < g.M && n < g.N) {
-                    const long long idx = c_offset(g, m, n);
-                    float v = alpha * acc[i][j][r];
-                    if (g.beta != 0.0) v += beta * C[idx];
-                    C[idx] = v;
-                }
-            }
+    store_c<float, TM, TN, 16>(
+        g, C, acc, [&](int i, int r) { return m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); },
+        [&](int j) { return n0 + wn * WN + j * 32 + (lane & 31); });
 }
 
 // ---------------------------------------------------------------------------
@@ -548,22 +553,9 @@ gemm_bf16k_kernel(GemmArgs g) {
         }
     }
 
-    const float alpha = (float)g.alpha, beta = (float)g.beta;
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int m = m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                const int n = n0 + wn * WN + j * 32 + (lane & 31);
-                if (m < g.M && n < g.N) {
-                    const long long idx = c_offset(g, m, n);
-                    float v = alpha * acc[i][j][r];
-                    if (g.beta != 0.0) v += beta * C[idx];
-                    C[idx] = v;
-                }
-            }
+    store_c<float, TM, TN, 16>(
+        g, C, acc, [&](int i, int r) { return m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); },
+        [&](int j) { return n0 + wn * WN + j * 32 + (lane & 31); });
 }
 
 // ---------------------------------------------------------------------------
