@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2, help="warm-up steps per precision tier")
     ap.add_argument("--chi", type=int, default=None, help="hotrg2d 64, hotrg3d 16, atrg3d 64")
     ap.add_argument("--nbeta", type=int, default=None, help="hotrg2d 32, hotrg3d 32 (of the 128-value draw)")
-    ap.add_argument("--fp64-steps", type=int, default=None, help="hotrg2d 6, hotrg3d 3")
+    ap.add_argument("--fp64-steps", type=int, default=None, help="hotrg2d 6, hotrg3d 4")
     ap.add_argument("--atrg-q", type=int, default=2)
     ap.add_argument("--atrg-prec", default="f32", choices=["f32", "f64"])
     ap.add_argument("--split-merge-from", type=int, default=10, help="bf16x3 merge after this step (-1: never)")
@@ -315,7 +315,7 @@ def main():
     args.chi = args.chi or {"hotrg2d": 64, "hotrg3d": 16, "atrg3d": 64}[wl]
     args.nbeta = args.nbeta or (1 if wl == "atrg3d" else 32)
     args.steps = args.steps or {"hotrg2d": 40, "hotrg3d": 6, "atrg3d": 30}[wl]
-    args.fp64_steps = args.fp64_steps if args.fp64_steps is not None else {"hotrg2d": 6, "hotrg3d": 3}.get(wl, 0)
+    args.fp64_steps = args.fp64_steps if args.fp64_steps is not None else {"hotrg2d": 6, "hotrg3d": 4}.get(wl, 0)
     if args.traffic_probe:
         return traffic_probe(args)
     if wl == "hotrg3d":
@@ -388,11 +388,10 @@ def main():
     torch.cuda.empty_cache()
 
     # merge GEMM launches grouped by (storage, compute)
-    names = {(_lib.F32, _lib.F32): "f32", (_lib.F32, _lib.BF16): "bf16", (_lib.F32, _lib.BF16X3): "bf16x3",
-             (_lib.F64, _lib.F64): "f64"}
+    names = {_lib.F32: "f32", _lib.BF16: "bf16", _lib.BF16X3: "bf16x3", _lib.F64: "f64"}
     groups = {}
     for (e0, e1, fl), cfg in zip(rec.events, rec.event_configs):
-        groups.setdefault(names[cfg[:2]], []).append((e0.elapsed_time(e1), fl, cfg))
+        groups.setdefault(names[cfg[2]], []).append((e0.elapsed_time(e1), fl, cfg))
     kern = {}
     for nm, evs in groups.items():
         ms = sum(e[0] for e in evs)
@@ -402,7 +401,7 @@ def main():
                     "flops_per_launch": fl / len(evs), "achieved_tflops": ach,
                     "peak_tflops": PEAK_TFLOPS[nm], "frac": ach / PEAK_TFLOPS[nm],
                     "mfma_issue_frac": ach * MFMA_PER_PRODUCT[nm] / PEAK_TFLOPS[nm],
-                    "tiles": sorted({autotune.config_name(c[0], c[1], c[2]) for _, _, c in evs})}
+                    "tiles": sorted({autotune.config_name(c[0], c[2], c[3], c[1]) for _, _, c in evs})}
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     d = kern[dom]
 

@@ -26,9 +26,10 @@ typedef enum {
     TRG_HIP_ERROR = 6     /* a HIP runtime call failed (see trg_last_hip_error) */
 } trg_status;
 
-/* Storage types (F32, F64; I32 for trg_fill only) and GEMM compute types
- * (F32, F64, BF16, BF16X3 = fp32 operands split into two bf16 parts, three
- * bf16 MFMAs per product, about 16 significant bits per operand). */
+/* Storage types (F32, F64, BF16 for GEMM operands / products and copies;
+ * I32 for trg_fill only) and GEMM compute types (F32, F64, BF16, BF16X3 =
+ * fp32 operands split into two bf16 parts, three bf16 MFMAs per product,
+ * about 16 significant bits per operand). */
 typedef enum { TRG_F32 = 0, TRG_F64 = 1, TRG_BF16 = 2, TRG_I32 = 3, TRG_BF16X3 = 4 } trg_dtype;
 
 const char *trg_version(void);
@@ -56,9 +57,13 @@ int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, double alph
              void *stream);
 
 /*
- * trg_gemm with an explicit tile configuration (0 <= config <
- * trg_gemm_config_count(dtype, compute); -1 = the shape heuristic of
- * trg_gemm) and an optional two-level output index: with cdm > 0 row m of
+ * trg_gemm with separate storage types for the operands (dtype, A and B) and
+ * the product (c_dtype), an explicit tile configuration (0 <= config <
+ * trg_gemm_config_count(dtype, c_dtype, compute); -1 = the shape heuristic)
+ * and an optional two-level output index.  Storage combinations:
+ * (F32, F32, F32|BF16|BF16X3), (F64, F64, F64), (BF16, F32, BF16) -- bf16
+ * operands, both K-contiguous with K and strides multiples of 8 -- and
+ * (F32, BF16, BF16), a product rounded to bf16 on the way out.  With cdm > 0 row m of
  * C[b] sits at (m / cdm) * csm + (m % cdm) * rsC, with cdn > 0 column n at
  * (n / cdn) * csn + (n % cdn) * csC (0 = the plain strides), so the product
  * lands directly in a permuted tensor layout.  All configurations of one
@@ -68,16 +73,17 @@ int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, double alph
  * selection behind the reference's cuquantum.contract calls
  * (trg/HOTRG_2d.py:26-27,65-67, trg/HOTRG_3d.py:42-45).
  */
-int trg_gemm_ex(int config, int dtype, int compute, int batch, int M, int N, int K, double alpha,
-                const void *A, long long sAb, long long rsA, long long csA,
+int trg_gemm_ex(int config, int dtype, int c_dtype, int compute, int batch, int M, int N, int K,
+                double alpha, const void *A, long long sAb, long long rsA, long long csA,
                 const void *B, long long sBb, long long rsB, long long csB,
                 double beta, void *C, long long sCb, long long rsC, long long csC,
                 int cdm, long long csm, int cdn, long long csn, void *stream);
-int trg_gemm_config_count(int dtype, int compute);
-const char *trg_gemm_config_name(int dtype, int compute, int config);
+int trg_gemm_config_count(int dtype, int c_dtype, int compute);
+const char *trg_gemm_config_name(int dtype, int c_dtype, int compute, int config);
 
 /*
- * Strided copy with optional F32<->F64 conversion over a rank-`rank` index
+ * Strided copy with optional F32<->F64 or F32<->BF16 conversion (round to
+ * nearest even) over a rank-`rank` index
  * space (rank <= 8):  out[sum i_k so[k]] = in[sum i_k si[k]].  Replaces the
  * leg permutations trg/HOTRG_2d.py:8-20, trg/HOTRG_3d.py:18-36,
  * trg/ATRG_3d_new.py:18-65 and the transpose-before-reshape of

@@ -187,7 +187,7 @@ def test_gemm_tile_configs_bitwise_equal(gpu, dtype, compute, shape, ta, tb):
     assert n >= 2
     for cfg in range(n):
         C = torch.empty_like(C0)
-        _lib.call("trg_gemm_ex", cfg, dt, cc, 3, M, N, K, 1.0, Av.data_ptr(), Av.stride(0), Av.stride(1),
+        _lib.call("trg_gemm_ex", cfg, dt, dt, cc, 3, M, N, K, 1.0, Av.data_ptr(), Av.stride(0), Av.stride(1),
                   Av.stride(2), Bv.data_ptr(), Bv.stride(0), Bv.stride(1), Bv.stride(2), 0.0, C.data_ptr(),
                   M * N, N, 1, 0, 0, 0, 0, ops.stream_ptr())
         assert torch.equal(C, C0), (cfg, autotune.config_name(dt, cc, cfg))
@@ -252,3 +252,39 @@ def test_gemm_two_level_output_index(gpu, compute):
     assert torch.equal(out.view(B, M2, N1, M1, N2), want)
     ops.gemm(A, Bm, out, beta=1.0, alpha=-1.0, compute=compute, out_layout=lay)
     assert float(out.abs().max()) == 0.0
+
+
+def test_bf16_storage_gemms_match_fp32_storage_bits(gpu):
+    """bf16 operands in memory give the bits of the fp32-storage bf16 GEMM (same rounding,
+    once, by the producer); a bf16 product is the fp32-accumulated product rounded once;
+    fp32 <-> bf16 copies round to nearest even like torch."""
+    from trg_backup_amd import autotune, ops
+    rng = np.random.default_rng(33)
+    B, M, N, K = 3, 200, 136, 96
+    A = _dev(rng.standard_normal((B, M, K)), gpu).float()                   # K-contiguous
+    Bt = _dev(rng.standard_normal((B, N, K)), gpu).float()                  # B(k,n) at n*K + k
+    Ab, Bb = ops.cast(A, torch.bfloat16), ops.cast(Bt, torch.bfloat16)
+    assert torch.equal(Ab, A.to(torch.bfloat16)) and torch.equal(ops.cast(Ab, torch.float32), Ab.float())
+    ref = ops.gemm(A, Bt.transpose(1, 2), compute="bf16")
+    for cfg in range(autotune.config_count(_lib_code("BF16"), _lib_code("BF16"), _lib_code("F32"))):
+        autotune.clear()
+        C = torch.empty_like(ref)
+        _gemm_cfg(cfg, Ab, Bb.transpose(1, 2), C)
+        assert torch.equal(C, ref), cfg
+    Cb = torch.empty((B, M, N), dtype=torch.bfloat16, device=gpu)
+    ops.gemm(A, Bt.transpose(1, 2), Cb, compute="bf16")
+    assert torch.equal(Cb, ref.to(torch.bfloat16))
+
+
+def _lib_code(name):
+    from trg_backup_amd import _lib
+    return getattr(_lib, name)
+
+
+def _gemm_cfg(cfg, A, B, C):
+    from trg_backup_amd import _lib, ops
+    bt, M, K = A.shape
+    N = B.shape[2]
+    _lib.call("trg_gemm_ex", cfg, ops.dtype_code(A), ops.dtype_code(C), _lib.BF16, bt, M, N, K, 1.0,
+              A.data_ptr(), A.stride(0), A.stride(1), A.stride(2), B.data_ptr(), B.stride(0), B.stride(1),
+              B.stride(2), 0.0, C.data_ptr(), M * N, N, 1, 0, 0, 0, 0, ops.stream_ptr())

@@ -31,16 +31,19 @@ def _cpu_gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_lay
     return None
 
 
-@pytest.mark.parametrize("shape", [(2, 4, 3, 5), (1, 6, 6, 6), (3, 5, 2, 4)])
+@pytest.mark.parametrize("shape", [(2, 4, 3, 5), (1, 6, 6, 6), (3, 5, 2, 4), (2, 4, 4, 6)])
 @pytest.mark.parametrize("compute", [None, "bf16"])
 def test_merge2d_chain_layouts(monkeypatch, shape, compute):
     B, D, Dy, chi = shape
     monkeypatch.setattr(hotrg, "gemm", _cpu_gemm)
-    monkeypatch.setattr(hotrg, "permute", lambda t, perm: t.permute(*perm).contiguous())
+    monkeypatch.setattr(hotrg, "permute", lambda t, perm, dtype=None: t.permute(*perm).contiguous().to(dtype or t.dtype))
     monkeypatch.setattr(hotrg, "INSTANCE_BUDGET", 2 * Dy * Dy * D * max(chi, Dy) * 8 * 2)   # 2 c per chunk
     g = torch.Generator().manual_seed(sum(shape))
     T = torch.randn(B, D, D, Dy, Dy, dtype=torch.float64, generator=g)
     U = torch.randn(B, D, D, chi, dtype=torch.float64, generator=g)
     ref = torch.einsum("zacke,zbdel,zabi,zcdj->zijkl", T, T, U, U)
     out = hotrg.merge2d(T, T, U, U, compute=compute)
-    assert (out - ref).abs().max().item() < 1e-10
+    # the bf16 path stores the chi^7 GEMM's operands in bfloat16 (a layout bug
+    # would show as an O(1) error, rounding as ~1e-2 relative)
+    tol = 3e-2 * ref.abs().max().item() if compute == "bf16" and (Dy * D) % 8 == 0 else 1e-10
+    assert (out - ref).abs().max().item() < tol

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtrg.so")
 
 F32, F64, BF16, I32, BF16X3 = 0, 1, 2, 3, 4
-ABI = 2          # must match the "abi N" revision in trg_version()
+ABI = 3          # must match the "abi N" revision in trg_version()
 
 c_int, c_ll, c_dbl, c_vp, c_ull, c_uint = (ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p,
                                            ctypes.c_ulonglong, ctypes.c_uint)
@@ -26,11 +26,11 @@ SIGNATURES = {
     "trg_gemm": [c_int, c_int, c_int, c_int, c_int, c_int, c_dbl,
                  c_vp, c_ll, c_ll, c_ll, c_vp, c_ll, c_ll, c_ll,
                  c_dbl, c_vp, c_ll, c_ll, c_ll, c_vp],
-    "trg_gemm_ex": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_dbl,
+    "trg_gemm_ex": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_dbl,
                     c_vp, c_ll, c_ll, c_ll, c_vp, c_ll, c_ll, c_ll,
                     c_dbl, c_vp, c_ll, c_ll, c_ll, c_int, c_ll, c_int, c_ll, c_vp],
-    "trg_gemm_config_count": [c_int, c_int],
-    "trg_gemm_config_name": [c_int, c_int, c_int],
+    "trg_gemm_config_count": [c_int, c_int, c_int],
+    "trg_gemm_config_name": [c_int, c_int, c_int, c_int],
     "trg_copy_strided": [c_int, c_int, c_int, P_LL, c_vp, P_LL, c_vp, P_LL, c_vp],
     "trg_fill": [c_int, c_ll, c_dbl, c_vp, c_vp],
     "trg_maxabs": [c_int, c_int, c_ll, c_vp, c_ll, c_vp, c_vp, c_int, c_vp],

@@ -218,9 +218,16 @@ def _truncate_with_tail(s, P, k):
 
 
 def squeezer(A, UM, sM, VMH, D, Dcut):
-    """:246-317 -> Px (x,x1,x0), PX (X1,X0,X), Py, PY."""
-    XdX, YdY, xxd, yyd = squeezer_environments(A, UM, sM, VMH, D)
+    """:246-317 -> Px (x,x1,x0), PX (X1,X0,X), Py, PY.
+
+    The environments are Gram matrices whose smallest kept eigenvalues sit
+    near 1e-8 of the largest at chi = 64, and the oblique projectors divide
+    by their square roots.  They are therefore always formed in float64 (24
+    chi^6 flop, small next to the chi^7 coarse graining): a float32 Gram is
+    not positive semi-definite at that level, and its negative noise
+    eigenvalues, amplified by s^-1/2, make a float32 run diverge."""
     dt = A.dtype
+    XdX, YdY, xxd, yyd = squeezer_environments(_f64(A), _f64(UM), sM, _f64(VMH), _f64(D))
     Px, PX = _oblique_pair(XdX, xxd, Dcut, (A.shape[1], UM.shape[1]), (VMH.shape[2], D.shape[2]), dt)
     Py, PY = _oblique_pair(YdY, yyd, Dcut, (A.shape[2], UM.shape[2]), (VMH.shape[3], D.shape[3]), dt)
     return Px, PX, Py, PY

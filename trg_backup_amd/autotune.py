@@ -54,18 +54,18 @@ def table():
         return dict(_TABLE)
 
 
-def config_count(dt, cc):
-    return _lib.load().trg_gemm_config_count(dt, cc)
+def config_count(dt, cc, cdt=None):
+    return _lib.load().trg_gemm_config_count(dt, dt if cdt is None else cdt, cc)
 
 
-def config_name(dt, cc, config):
+def config_name(dt, cc, config, cdt=None):
     if config < 0:
         return "heuristic"
-    return _lib.load().trg_gemm_config_name(dt, cc, config).decode()
+    return _lib.load().trg_gemm_config_name(dt, dt if cdt is None else cdt, cc, config).decode()
 
 
-def _key(dt, cc, batch, M, N, K, amc, bnc, aligned):
-    return f"{dt}:{cc}:{batch}:{M}:{N}:{K}:{int(amc)}{int(bnc)}:{int(aligned)}"
+def _key(dt, cdt, cc, batch, M, N, K, amc, bnc, aligned):
+    return f"{dt}:{cdt}:{cc}:{batch}:{M}:{N}:{K}:{int(amc)}{int(bnc)}:{int(aligned)}"
 
 
 def save(path):
@@ -90,30 +90,31 @@ def _aligned(args):
     return ok(A, sA, rsA if rsA != 1 else csA) and ok(B, sB, rsB if rsB != 1 else csB)
 
 
-def choose(dt, cc, batch, M, N, K, operands, launch, out=None):
+def choose(dt, cc, batch, M, N, K, operands, launch, out=None, cdt=None):
     """Config for this problem (-1 = heuristic).  `launch(config, C)` runs
     the GEMM with beta = 0 into a (batch, M, N) contiguous buffer C; it is
     called only while tuning, into `out` when the caller's output may be
     overwritten (beta = 0, contiguous), else into a scratch buffer."""
     A, sA, rsA, csA, B, sB, rsB, csB = operands
-    key = _key(dt, cc, batch, M, N, K, rsA == 1, csB == 1, _aligned(operands))
+    cdt = dt if cdt is None else cdt
+    key = _key(dt, cdt, cc, batch, M, N, K, rsA == 1, csB == 1, _aligned(operands))
     with _LOCK:
         hit = _TABLE.get(key)
     if hit is not None or not _STATE["enabled"]:
         return -1 if hit is None else hit
     if torch.cuda.is_current_stream_capturing():
         return -1
-    best = _tune(dt, cc, batch, M, N, K, launch, out)
+    best = _tune(dt, cdt, cc, batch, M, N, K, launch, out)
     with _LOCK:
         _TABLE[key] = best
     return best
 
 
-def _tune(dt, cc, batch, M, N, K, launch, out):
-    n = config_count(dt, cc)
+def _tune(dt, cdt, cc, batch, M, N, K, launch, out):
+    n = config_count(dt, cc, cdt)
     if n <= 1:
         return -1
-    dtype = torch.float64 if dt == _lib.F64 else torch.float32
+    dtype = {_lib.F64: torch.float64, _lib.BF16: torch.bfloat16}.get(cdt, torch.float32)
     scratch = out if out is not None else torch.empty((batch, M, N), dtype=dtype,
                                                        device=torch.cuda.current_device())
     best, best_ms = -1, float("inf")

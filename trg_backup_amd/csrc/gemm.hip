@@ -52,9 +52,9 @@ __device__ inline long long c_col_offset(const GemmArgs &g, int n) {
 // once per accumulator row, so no index division sits in the inner loop.
 // row(i, r) / col(j) give the C row of accumulator register r of block i and
 // the C column of block j for this lane.
-template <typename T, int TM, int TN, int NACC, typename Acc, typename RowF, typename ColF>
-__device__ inline void store_c(const GemmArgs &g, T *__restrict__ C, Acc (&acc)[TM][TN], RowF row, ColF col) {
-    const T alpha = (T)g.alpha, beta = (T)g.beta;
+template <typename TA, typename TC, int TM, int TN, int NACC, typename Acc, typename RowF, typename ColF>
+__device__ inline void store_c(const GemmArgs &g, TC *__restrict__ C, Acc (&acc)[TM][TN], RowF row, ColF col) {
+    const TA alpha = (TA)g.alpha, beta = (TA)g.beta;
     long long on[TN];
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
@@ -72,9 +72,9 @@ __device__ inline void store_c(const GemmArgs &g, T *__restrict__ C, Acc (&acc)[
             for (int j = 0; j < TN; ++j) {
                 if (on[j] < 0) continue;
                 const long long idx = om + on[j];
-                T v = alpha * acc[i][j][r];
-                if (g.beta != 0.0) v += beta * C[idx];
-                C[idx] = v;
+                TA v = alpha * acc[i][j][r];
+                if (g.beta != 0.0) v += beta * (TA)C[idx];
+                C[idx] = (TC)v;
             }
         }
 }
@@ -279,7 +279,7 @@ gemm_kernel(GemmArgs g) {
         }
     }
 
-    store_c<T, TM, TN, MF::NACC>(
+    store_c<T, T, TM, TN, MF::NACC>(
         g, C, acc, [&](int i, int r) { return m0 + wm * WM + i * MF::MB + MF::c_row(lane, r); },
         [&](int j) { return n0 + wn * WN + j * MF::MB + MF::c_col(lane); });
 }
@@ -341,7 +341,7 @@ __device__ inline void bf16_slice(const __bf16 *As, const __bf16 *Bs, f32x16 (&a
     }
 }
 
-template <int BM, int BN, int BK, int WM, int WN, bool A_MCONT, bool B_NCONT, bool SPLIT>
+template <int BM, int BN, int BK, int WM, int WN, bool A_MCONT, bool B_NCONT, bool SPLIT, typename TC>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
 gemm_bf16_kernel(GemmArgs g) {
     constexpr int WAVES_M = BM / WM;
@@ -363,7 +363,7 @@ gemm_bf16_kernel(GemmArgs g) {
     const int m0 = tm * BM, n0 = tn * BN;
     const float *__restrict__ A = static_cast<const float *>(g.A) + (long long)b * g.sAb;
     const float *__restrict__ B = static_cast<const float *>(g.B) + (long long)b * g.sBb;
-    float *__restrict__ C = static_cast<float *>(g.C) + (long long)b * g.sCb;
+    TC *__restrict__ C = static_cast<TC *>(g.C) + (long long)b * g.sCb;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WAVES_M, wn = wave / WAVES_M;
@@ -428,7 +428,7 @@ gemm_bf16_kernel(GemmArgs g) {
         }
     }
 
-    store_c<float, TM, TN, 16>(
+    store_c<float, TC, TM, TN, 16>(
         g, C, acc, [&](int i, int r) { return m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); },
         [&](int j) { return n0 + wn * WN + j * 32 + (lane & 31); });
 }
@@ -449,7 +449,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // DB: two LDS stages; the next slice is written into the stage the waves are
 // not reading, so one barrier per K slice suffices instead of two.
-template <int BM, int BN, int BK, int WM, int WN, bool SPLIT, bool DB>
+template <int BM, int BN, int BK, int WM, int WN, bool SPLIT, bool DB, typename TC>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
 gemm_bf16k_kernel(GemmArgs g) {
     constexpr int WAVES_M = BM / WM;
@@ -475,7 +475,7 @@ gemm_bf16k_kernel(GemmArgs g) {
     const int m0 = tm * BM, n0 = tn * BN;
     const float *__restrict__ A = static_cast<const float *>(g.A) + (long long)b * g.sAb;
     const float *__restrict__ B = static_cast<const float *>(g.B) + (long long)b * g.sBb;
-    float *__restrict__ C = static_cast<float *>(g.C) + (long long)b * g.sCb;
+    TC *__restrict__ C = static_cast<TC *>(g.C) + (long long)b * g.sCb;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WAVES_M, wn = wave / WAVES_M;
@@ -553,7 +553,109 @@ gemm_bf16k_kernel(GemmArgs g) {
         }
     }
 
-    store_c<float, TM, TN, 16>(
+    store_c<float, TC, TM, TN, 16>(
+        g, C, acc, [&](int i, int r) { return m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); },
+        [&](int j) { return n0 + wn * WN + j * 32 + (lane & 31); });
+}
+
+// ---------------------------------------------------------------------------
+// bf16 storage, bf16 MFMA, fp32 output: both operands bf16 and K-contiguous,
+// A(m,k) at m*rsA + k and B(k,n) at n*csB + k.  Each thread moves 16-byte
+// runs of 8 consecutive k straight from global memory into the k-fastest LDS
+// rows, so an operand costs half the L2 traffic of the fp32-storage kernel
+// and no conversion.  Requires 16-byte aligned bases and K, rsA, csB, sAb,
+// sBb multiples of 8.  Bitwise equal to gemm_bf16k_kernel on operands that
+// were rounded to bf16 beforehand (same MFMA, same k order).
+// ---------------------------------------------------------------------------
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+template <int BM, int BN, int BK, int WM, int WN, bool DB>
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
+gemm_bf16in_kernel(GemmArgs g) {
+    constexpr int WAVES_M = BM / WM;
+    constexpr int NT = (BM / WM) * (BN / WN) * 64;
+    constexpr int TM = WM / 32;
+    constexpr int TN = WN / 32;
+    constexpr int KP = BK + 8;
+    constexpr int KQ = BK / 8;            // bf16 vectors per tile row
+    constexpr int AV = BM * KQ / NT;
+    constexpr int BV = BN * KQ / NT;
+    static_assert(AV * NT == BM * KQ && BV * NT == BN * KQ, "tile not divisible by threads");
+    static_assert(BK % 16 == 0, "BK must be a multiple of 16");
+    constexpr int NB = DB ? 2 : 1;
+    constexpr int ASZ = BM * KP, BSZ = BN * KP;
+    __shared__ __attribute__((aligned(16))) __bf16 As[NB * ASZ];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[NB * BSZ];
+
+    int tm, tn;
+    tile_coords(g, tm, tn);
+    const int b = blockIdx.y;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const __bf16 *__restrict__ A = static_cast<const __bf16 *>(g.A) + (long long)b * g.sAb;
+    const __bf16 *__restrict__ B = static_cast<const __bf16 *>(g.B) + (long long)b * g.sBb;
+    float *__restrict__ C = static_cast<float *>(g.C) + (long long)b * g.sCb;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    u16x8 ra[AV], rb[BV];
+    const u16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < AV; ++i) {
+            const int e = tid + i * NT;
+            const int m = m0 + e / KQ, k = k0 + 8 * (e % KQ);
+            ra[i] = (m < g.M && k < g.K) ? *reinterpret_cast<const u16x8 *>(A + (long long)m * g.rsA + k) : zero;
+        }
+#pragma unroll
+        for (int i = 0; i < BV; ++i) {
+            const int e = tid + i * NT;
+            const int n = n0 + e / KQ, k = k0 + 8 * (e % KQ);
+            rb[i] = (n < g.N && k < g.K) ? *reinterpret_cast<const u16x8 *>(B + (long long)n * g.csB + k) : zero;
+        }
+    };
+    auto store = [&](int stage) {
+#pragma unroll
+        for (int i = 0; i < AV; ++i) {
+            const int e = tid + i * NT;
+            *reinterpret_cast<u16x8 *>(&As[stage * ASZ + (e / KQ) * KP + 8 * (e % KQ)]) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < BV; ++i) {
+            const int e = tid + i * NT;
+            *reinterpret_cast<u16x8 *>(&Bs[stage * BSZ + (e / KQ) * KP + 8 * (e % KQ)]) = rb[i];
+        }
+    };
+
+    load(0);
+    store(0);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        const bool more = k0 + BK < g.K;
+        if (more) load(k0 + BK);
+        bf16_slice<BM, BN, BK, KP, WM, WN, false>(As + cur * ASZ, Bs + cur * BSZ, acc, wm, wn, lane);
+        if constexpr (DB) {
+            if (more) store(cur ^ 1);
+            __syncthreads();
+            cur ^= 1;
+        } else {
+            __syncthreads();
+            if (more) {
+                store(0);
+                __syncthreads();
+            }
+        }
+    }
+    store_c<float, float, TM, TN, 16>(
         g, C, acc, [&](int i, int r) { return m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); },
         [&](int j) { return n0 + wn * WN + j * 32 + (lane & 31); });
 }
@@ -603,13 +705,14 @@ static bool kvec_ok(const GemmArgs &g, bool amc, bool bnc) {
 // KONLY: the tile exists for the K-contiguous kernel only (its generic
 // counterpart would spill registers); other layouts run tile 256x128x32 w64x64
 // (128x128x32 w64x64 when SPLIT), which gives the same bits.
-template <int BM, int BN, int BK, int WM, int WN, bool SPLIT, bool KONLY = false, bool DB = false>
+template <int BM, int BN, int BK, int WM, int WN, bool SPLIT, bool KONLY = false, bool DB = false,
+          typename TC = float>
 static int launch_bf16(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
     static_assert(!DB || KONLY, "double-buffered tiles exist for the K-contiguous kernel only");
     if constexpr (KONLY) {
         if (!kvec_ok(g, amc, bnc)) {
-            if constexpr (SPLIT) return launch_bf16<128, 128, 32, 64, 64, true>(g, batch, amc, bnc, s);
-            else return launch_bf16<256, 128, 32, 64, 64, false>(g, batch, amc, bnc, s);
+            if constexpr (SPLIT) return launch_bf16<128, 128, 32, 64, 64, true, false, false, TC>(g, batch, amc, bnc, s);
+            else return launch_bf16<256, 128, 32, 64, 64, false, false, false, TC>(g, batch, amc, bnc, s);
         }
     }
     g.tiles_m = (int)cdiv(g.M, BM);
@@ -617,28 +720,72 @@ static int launch_bf16(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s)
     dim3 grid(g.tiles_m * g.tiles_n, batch);
     dim3 block((BM / WM) * (BN / WN) * 64);
     if (KONLY || kvec_ok(g, amc, bnc)) {
-        hipLaunchKernelGGL((gemm_bf16k_kernel<BM, BN, BK, WM, WN, SPLIT, DB>), grid, block, 0, s, g);
+        hipLaunchKernelGGL((gemm_bf16k_kernel<BM, BN, BK, WM, WN, SPLIT, DB, TC>), grid, block, 0, s, g);
         return check_launch();
     }
     if constexpr (!KONLY) {
         if (amc && bnc)
-            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, true, SPLIT>), grid, block, 0, s, g);
+            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, true, SPLIT, TC>), grid, block, 0, s, g);
         else if (amc)
-            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, false, SPLIT>), grid, block, 0, s, g);
+            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, true, false, SPLIT, TC>), grid, block, 0, s, g);
         else if (bnc)
-            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, true, SPLIT>), grid, block, 0, s, g);
+            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, true, SPLIT, TC>), grid, block, 0, s, g);
         else
-            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, false, SPLIT>), grid, block, 0, s, g);
+            hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, BK, WM, WN, false, false, SPLIT, TC>), grid, block, 0, s, g);
     }
+    return check_launch();
+}
+
+// bf16 operands in memory: both must be K-contiguous with 16-byte runs of 8
+// elements that never straddle K.
+static bool kvec8_ok(const GemmArgs &g, bool amc, bool bnc) {
+    if (amc || bnc) return false;
+    auto al = [](const void *p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
+    return al(g.A) && al(g.B) && g.K % 8 == 0 && g.rsA % 8 == 0 && g.csB % 8 == 0 && g.sAb % 8 == 0 &&
+           g.sBb % 8 == 0;
+}
+
+template <int BM, int BN, int BK, int WM, int WN, bool DB = false>
+static int launch_bf16in(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
+    if (!kvec8_ok(g, amc, bnc)) return TRG_UNSUPPORTED;
+    g.tiles_m = (int)cdiv(g.M, BM);
+    g.tiles_n = (int)cdiv(g.N, BN);
+    dim3 grid(g.tiles_m * g.tiles_n, batch);
+    dim3 block((BM / WM) * (BN / WN) * 64);
+    hipLaunchKernelGGL((gemm_bf16in_kernel<BM, BN, BK, WM, WN, DB>), grid, block, 0, s, g);
     return check_launch();
 }
 
 // Tile configuration table.  config -1 selects by shape (the default
 // heuristic).  Every fp32 configuration uses the same MFMA instruction and
 // walks K in the same order, so the choice never changes the result bits.
-static int gemm_dispatch(int dtype, int compute, int config, GemmArgs g, int batch, bool amc, bool bnc,
-                         hipStream_t s) {
+static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmArgs g, int batch, bool amc,
+                         bool bnc, hipStream_t s) {
     const long long work = (long long)g.M * g.N;
+    if (dtype == TRG_BF16 && c_dtype == TRG_F32 && compute == TRG_BF16) {
+        if (config < 0) config = work >= 256LL * 256 * 64 ? 3 : 1;
+        switch (config) {
+            case 0: return launch_bf16in<128, 128, 32, 64, 64>(g, batch, amc, bnc, s);
+            case 1: return launch_bf16in<64, 64, 32, 32, 32>(g, batch, amc, bnc, s);
+            case 2: return launch_bf16in<256, 128, 32, 64, 64>(g, batch, amc, bnc, s);
+            case 3: return launch_bf16in<256, 256, 32, 128, 64>(g, batch, amc, bnc, s);
+            case 4: return launch_bf16in<256, 256, 64, 128, 64>(g, batch, amc, bnc, s);
+            case 5: return launch_bf16in<256, 256, 32, 128, 64, true>(g, batch, amc, bnc, s);
+            case 6: return launch_bf16in<256, 128, 64, 64, 64, true>(g, batch, amc, bnc, s);
+            case 7: return launch_bf16in<128, 256, 32, 64, 64>(g, batch, amc, bnc, s);
+            default: return TRG_BAD_ARG;
+        }
+    }
+    if (dtype == TRG_F32 && c_dtype == TRG_BF16 && compute == TRG_BF16) {
+        if (config < 0) config = work >= 256LL * 128 * 128 ? 0 : 1;
+        switch (config) {
+            case 0: return launch_bf16<128, 128, 32, 64, 64, false, false, false, __bf16>(g, batch, amc, bnc, s);
+            case 1: return launch_bf16<64, 64, 32, 32, 32, false, false, false, __bf16>(g, batch, amc, bnc, s);
+            case 2: return launch_bf16<256, 128, 32, 64, 64, false, false, false, __bf16>(g, batch, amc, bnc, s);
+            default: return TRG_BAD_ARG;
+        }
+    }
+    if (c_dtype != dtype) return TRG_UNSUPPORTED;
     if (dtype == TRG_F32 && compute == TRG_F32) {
         if (config < 0) {
             if (g.N <= 64 && g.M >= 4096) config = 3;
@@ -707,7 +854,7 @@ static int gemm_dispatch(int dtype, int compute, int config, GemmArgs g, int bat
 
 using namespace trg;
 
-extern "C" int trg_gemm_config_count(int dtype, int compute);
+extern "C" int trg_gemm_config_count(int dtype, int c_dtype, int compute);
 
 extern "C" int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, double alpha,
                         const void *A, long long sAb, long long rsA, long long csA,
@@ -727,10 +874,14 @@ extern "C" int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, 
         // C = beta * C: a GEMM with an empty sum.  Use the kernel with K = 0 (loop skipped).
     }
     GemmArgs g{M, N, K, A, sAb, rsA, csA, B, sBb, rsB, csB, C, sCb, rsC, csC, alpha, beta, 0, 0};
-    return gemm_dispatch(dtype, compute, -1, g, batch, amc, bnc, s);
+    if (dtype == TRG_BF16) return TRG_UNSUPPORTED;   // bf16 storage: trg_gemm_ex
+    return gemm_dispatch(dtype, dtype, compute, -1, g, batch, amc, bnc, s);
 }
 
-extern "C" int trg_gemm_config_count(int dtype, int compute) {
+extern "C" int trg_gemm_config_count(int dtype, int c_dtype, int compute) {
+    if (dtype == TRG_BF16 && c_dtype == TRG_F32 && compute == TRG_BF16) return 8;
+    if (dtype == TRG_F32 && c_dtype == TRG_BF16 && compute == TRG_BF16) return 3;
+    if (c_dtype != dtype) return 0;
     if (dtype == TRG_F32 && compute == TRG_F32) return 8;
     if (dtype == TRG_F32 && compute == TRG_BF16) return 12;
     if (dtype == TRG_F32 && compute == TRG_BF16X3) return 8;
@@ -738,7 +889,7 @@ extern "C" int trg_gemm_config_count(int dtype, int compute) {
     return 0;
 }
 
-extern "C" const char *trg_gemm_config_name(int dtype, int compute, int config) {
+extern "C" const char *trg_gemm_config_name(int dtype, int c_dtype, int compute, int config) {
     static const char *f32[] = {"128x128x32 w64x64", "128x128x16 w64x64", "64x64x16 w32x32", "256x64x16 w64x64",
                                 "64x256x16 w64x64", "256x128x32 w64x64", "128x256x32 w64x64", "256x128x16 w128x64"};
     static const char *bf[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "256x128x32 w128x64",
@@ -748,16 +899,21 @@ extern "C" const char *trg_gemm_config_name(int dtype, int compute, int config) 
     static const char *bx3[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "128x256x32 w64x64",
                                 "256x256x32 w128x64", "256x128x32 w128x64", "256x128x32 w64x64 db",
                                 "128x256x32 w64x64 db"};
+    static const char *bin[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "256x256x32 w128x64",
+                                "256x256x64 w128x64", "256x256x32 w128x64 db", "256x128x64 w64x64 db",
+                                "128x256x32 w64x64"};
     static const char *f64[] = {"128x128x16 w64x64", "64x64x16 w32x32", "128x64x16 w64x32", "64x128x16 w32x64"};
-    if (config < 0 || config >= trg_gemm_config_count(dtype, compute)) return "";
+    if (config < 0 || config >= trg_gemm_config_count(dtype, c_dtype, compute)) return "";
+    if (dtype == TRG_BF16) return bin[config];
+    if (c_dtype == TRG_BF16) return bf[config];
     if (dtype == TRG_F32 && compute == TRG_F32) return f32[config];
     if (dtype == TRG_F32 && compute == TRG_BF16) return bf[config];
     if (dtype == TRG_F32 && compute == TRG_BF16X3) return bx3[config];
     return f64[config];
 }
 
-extern "C" int trg_gemm_ex(int config, int dtype, int compute, int batch, int M, int N, int K, double alpha,
-                           const void *A, long long sAb, long long rsA, long long csA,
+extern "C" int trg_gemm_ex(int config, int dtype, int c_dtype, int compute, int batch, int M, int N, int K,
+                           double alpha, const void *A, long long sAb, long long rsA, long long csA,
                            const void *B, long long sBb, long long rsB, long long csB,
                            double beta, void *C, long long sCb, long long rsC, long long csC,
                            int cdm, long long csm, int cdn, long long csn, void *stream) {
@@ -765,12 +921,13 @@ extern "C" int trg_gemm_ex(int config, int dtype, int compute, int batch, int M,
     if (batch == 0 || M == 0 || N == 0) return TRG_OK;
     if (!A || !B || !C) return TRG_BAD_ARG;
     if (batch > 65535) return TRG_SHAPE;
-    if (config < -1 || config >= trg_gemm_config_count(dtype, compute)) return TRG_BAD_ARG;
+    if (trg_gemm_config_count(dtype, c_dtype, compute) == 0) return TRG_UNSUPPORTED;
+    if (config < -1 || config >= trg_gemm_config_count(dtype, c_dtype, compute)) return TRG_BAD_ARG;
     const bool amc = (rsA == 1);
     const bool bnc = (csB == 1);
     if (!amc && csA != 1) return TRG_BAD_ARG;
     if (!bnc && rsB != 1) return TRG_BAD_ARG;
     GemmArgs g{M, N, K, A, sAb, rsA, csA, B, sBb, rsB, csB, C, sCb, rsC, csC, alpha, beta, 0, 0,
                cdm, cdn, csm, csn};
-    return gemm_dispatch(dtype, compute, config, g, batch, amc, bnc, as_stream(stream));
+    return gemm_dispatch(dtype, c_dtype, compute, config, g, batch, amc, bnc, as_stream(stream));
 }

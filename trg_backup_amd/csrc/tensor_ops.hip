@@ -366,6 +366,9 @@ extern "C" int trg_copy_strided(int dtype_in, int dtype_out, int rank, const lon
             else if (dtype_in == TRG_F64 && dtype_out == TRG_F64) hipLaunchKernelGGL((rowcopy_kernel<double, double>), grid, 256, 0, s, r, (const double *)in, (double *)out, nblk);
             else if (dtype_in == TRG_F32 && dtype_out == TRG_F64) hipLaunchKernelGGL((rowcopy_kernel<float, double>), grid, 256, 0, s, r, (const float *)in, (double *)out, nblk);
             else if (dtype_in == TRG_F64 && dtype_out == TRG_F32) hipLaunchKernelGGL((rowcopy_kernel<double, float>), grid, 256, 0, s, r, (const double *)in, (float *)out, nblk);
+            else if (dtype_in == TRG_F32 && dtype_out == TRG_BF16) hipLaunchKernelGGL((rowcopy_kernel<float, __bf16>), grid, 256, 0, s, r, (const float *)in, (__bf16 *)out, nblk);
+            else if (dtype_in == TRG_BF16 && dtype_out == TRG_F32) hipLaunchKernelGGL((rowcopy_kernel<__bf16, float>), grid, 256, 0, s, r, (const __bf16 *)in, (float *)out, nblk);
+            else if (dtype_in == TRG_BF16 && dtype_out == TRG_BF16) hipLaunchKernelGGL((rowcopy_kernel<__bf16, __bf16>), grid, 256, 0, s, r, (const __bf16 *)in, (__bf16 *)out, nblk);
             else return TRG_UNSUPPORTED;
             return check_launch();
         }
@@ -392,6 +395,9 @@ extern "C" int trg_copy_strided(int dtype_in, int dtype_out, int rank, const lon
             else if (dtype_in == TRG_F64 && dtype_out == TRG_F64) hipLaunchKernelGGL((transpose_kernel<double, double>), grid, 256, 0, s, t, (const double *)in, (double *)out);
             else if (dtype_in == TRG_F32 && dtype_out == TRG_F64) hipLaunchKernelGGL((transpose_kernel<float, double>), grid, 256, 0, s, t, (const float *)in, (double *)out);
             else if (dtype_in == TRG_F64 && dtype_out == TRG_F32) hipLaunchKernelGGL((transpose_kernel<double, float>), grid, 256, 0, s, t, (const double *)in, (float *)out);
+            else if (dtype_in == TRG_F32 && dtype_out == TRG_BF16) hipLaunchKernelGGL((transpose_kernel<float, __bf16>), grid, 256, 0, s, t, (const float *)in, (__bf16 *)out);
+            else if (dtype_in == TRG_BF16 && dtype_out == TRG_F32) hipLaunchKernelGGL((transpose_kernel<__bf16, float>), grid, 256, 0, s, t, (const __bf16 *)in, (float *)out);
+            else if (dtype_in == TRG_BF16 && dtype_out == TRG_BF16) hipLaunchKernelGGL((transpose_kernel<__bf16, __bf16>), grid, 256, 0, s, t, (const __bf16 *)in, (__bf16 *)out);
             else return TRG_UNSUPPORTED;
             return check_launch();
         }
@@ -405,6 +411,12 @@ extern "C" int trg_copy_strided(int dtype_in, int dtype_out, int rank, const lon
         hipLaunchKernelGGL((copy_strided_kernel<float, double>), g, 256, 0, s, d, (const float *)in, (double *)out, total);
     else if (dtype_in == TRG_F64 && dtype_out == TRG_F32)
         hipLaunchKernelGGL((copy_strided_kernel<double, float>), g, 256, 0, s, d, (const double *)in, (float *)out, total);
+    else if (dtype_in == TRG_F32 && dtype_out == TRG_BF16)
+        hipLaunchKernelGGL((copy_strided_kernel<float, __bf16>), g, 256, 0, s, d, (const float *)in, (__bf16 *)out, total);
+    else if (dtype_in == TRG_BF16 && dtype_out == TRG_F32)
+        hipLaunchKernelGGL((copy_strided_kernel<__bf16, float>), g, 256, 0, s, d, (const __bf16 *)in, (float *)out, total);
+    else if (dtype_in == TRG_BF16 && dtype_out == TRG_BF16)
+        hipLaunchKernelGGL((copy_strided_kernel<__bf16, __bf16>), g, 256, 0, s, d, (const __bf16 *)in, (__bf16 *)out, total);
     else
         return TRG_UNSUPPORTED;
     return check_launch();

@@ -198,8 +198,12 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
     # keeps its innermost index contiguous in the destination (coalesced
     # epilogue stores).
     kinner = compute in ("bf16", "bf16x3")
+    # plain bf16: the two operands of the chi^7 GEMM are stored in bf16 (the
+    # rounding the MFMA would apply anyway, done once by the producer), which
+    # halves the operand traffic of the L2-fed bf16 kernel; same bits
+    zdt = torch.bfloat16 if compute == "bf16" and (Dy * Dx) % 8 == 0 else T1.dtype
     if kinner:
-        Tp = permute(T2, (0, 4, 2, 3, 1)).view(B, Dy * Dx, Dy * Dx).transpose(1, 2)   # [(e,b),(l,d)]
+        Tp = permute(T2, (0, 4, 2, 3, 1), zdt).view(B, Dy * Dx, Dy * Dx).transpose(1, 2)   # [(e,b),(l,d)]
         Uam = permute(Ua, (0, 1, 3, 2)).view(B, Dx, chi_a * Dx)                     # [a,(i,b)]
     else:
         Tp = permute(T2, (0, 3, 1, 4, 2)).view(B, Dy * Dx, Dy * Dx)      # [(e,b),(l,d)]
@@ -212,7 +216,7 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
         c1 = min(Dx, c0 + cc)
         nc = c1 - c0
         A = T1m[:, :, c0 * Dy * Dy:c1 * Dy * Dy].transpose(1, 2)          # (B, (c,k,e), a)
-        Z1p = torch.empty(B * nc * Dy * Dy * Dx * chi_a, dtype=T1.dtype, device=T1.device)
+        Z1p = torch.empty(B * nc * Dy * Dy * Dx * chi_a, dtype=zdt, device=T1.device)
         if kinner:
             # [c,k,i,e,b]: rows (c,k | e), columns (i | b)
             gemm(A, Uam, Z1p, compute=compute,

@@ -10,14 +10,14 @@ import torch
 from . import _lib, autotune
 from ._lib import BF16, BF16X3, F32, F64, I32, call, ll_array
 
-_DT = {torch.float32: F32, torch.float64: F64}
+_DT = {torch.float32: F32, torch.float64: F64, torch.bfloat16: BF16}
 
 
 def dtype_code(t):
     try:
         return _DT[t.dtype]
     except KeyError:
-        raise _lib.TrgError(f"unsupported dtype {t.dtype}; libtrg works on float32/float64") from None
+        raise _lib.TrgError(f"unsupported dtype {t.dtype}; libtrg works on float32/float64/bfloat16") from None
 
 
 def require_device(*ts):
@@ -108,7 +108,7 @@ class GemmRecorder:
     def __init__(self, tag):
         self.tag = tag
         self.events = []   # (start, end, flops)
-        self.event_configs = []   # (dtype, compute, tile config, M, N, K, batch) per event
+        self.event_configs = []   # (dtype, C dtype, compute, tile config, M, N, K, batch) per event
 
     def __enter__(self):
         _RECORDERS.append(self)
@@ -170,7 +170,7 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_layout=N
     if C is None:
         C = torch.empty((batch, M, N), dtype=A.dtype, device=A.device)
         beta = 0.0
-    if A.dtype != B.dtype or A.dtype != C.dtype:
+    if A.dtype != B.dtype:
         raise _lib.TrgError("gemm operands must share a dtype")
     sA = A.stride(0) if bt > 1 else 0
     sB = B.stride(0) if bt2 > 1 else 0
@@ -189,9 +189,9 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_layout=N
     if rsB != 1 and csB != 1:
         B = contiguous(B)
         rsB, csB, sB = B.stride(1), B.stride(2), (B.stride(0) if bt2 > 1 else 0)
-    dt = dtype_code(A)
+    dt, cdt = dtype_code(A), dtype_code(C)
     if compute is None:
-        cc = dt
+        cc = BF16 if dt == BF16 else dt
     elif compute == "bf16":
         cc = BF16
     elif compute == "bf16x3":
@@ -210,21 +210,21 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_layout=N
     config = -1
     if autotune.active():
         def launch(cfg, out):
-            call("trg_gemm_ex", cfg, dt, cc, batch, M, N, K, float(alpha), *opnds,
+            call("trg_gemm_ex", cfg, dt, cdt, cc, batch, M, N, K, float(alpha), *opnds,
                  0.0, ptr(out), M * N if batch > 1 else 0, N, 1, 0, 0, 0, 0, stream_ptr())
         own = out_layout is None and beta == 0.0 and C.is_contiguous() and C.shape == (batch, M, N)
-        config = autotune.choose(dt, cc, batch, M, N, K, opnds, launch, C if own else None)
+        config = autotune.choose(dt, cc, batch, M, N, K, opnds, launch, C if own else None, cdt=cdt)
     recs = [r for r in _RECORDERS if tag is not None and r.tag == tag]
     if recs:
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record()
-    call("trg_gemm_ex", config, dt, cc, batch, M, N, K, float(alpha), *opnds,
+    call("trg_gemm_ex", config, dt, cdt, cc, batch, M, N, K, float(alpha), *opnds,
          float(beta), ptr(C), *Cstr, *split, stream_ptr())
     if recs:
         ev1.record()
         for r in recs:
             r.events.append((ev0, ev1, 2.0 * M * N * K * batch))
-            r.event_configs.append((dt, cc, config, M, N, K, batch))
+            r.event_configs.append((dt, cdt, cc, config, M, N, K, batch))
     if out_layout is not None:
         return None
     return C if batched else C[0]
