@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2, help="warm-up steps per precision tier")
     ap.add_argument("--chi", type=int, default=None, help="hotrg2d 64, hotrg3d 16, atrg3d 64")
     ap.add_argument("--nbeta", type=int, default=None, help="hotrg2d 32, hotrg3d 32 (of the 128-value draw)")
-    ap.add_argument("--fp64-steps", type=int, default=None, help="hotrg2d 6, hotrg3d 4")
+    ap.add_argument("--fp64-steps", type=int, default=None, help="hotrg2d 6, hotrg3d 4, atrg3d 9")
     ap.add_argument("--atrg-q", type=int, default=2)
     ap.add_argument("--atrg-prec", default="f32", choices=["f32", "f64"])
     ap.add_argument("--split-merge-from", type=int, default=10, help="bf16x3 merge after this step (-1: never)")
@@ -72,6 +72,7 @@ def parse():
     ap.add_argument("--serve-chis", default="16,24,32", help="bond dimensions of the request stream")
     ap.add_argument("--serve-steps", default="20,30", help="RG-step range of the request stream")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="hotrg3d: eager steady steps instead of HIP graphs")
     ap.add_argument("--load-tune", default=None, help="GEMM tile table (JSON) to start from")
     ap.add_argument("--save-tune", default=None, help="write the GEMM tile table after the warm-up")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -137,16 +138,48 @@ def bench_hotrg3d(args):
     autotune.enable(False)
     log(f"warm-up in {time.perf_counter() - t0:.1f} s")
     torch.cuda.empty_cache()
-    rec = ops.GemmRecorder("merge3d_x3")
+    graphed = {}
+    if not args.no_graphs and K > args.fp64_steps:
+        # one captured batched launch sequence per steady direction (float32 steps)
+        from trg_backup_amd import graphs
+        counter = graphs.step_counter(dev)
+        lnf_g = ops.fill_(torch.empty((nb, K + 1), dtype=torch.float64, device=dev), 0.0)
+        for d in sorted({dd for n, dd in sched if n > args.fp64_steps}):
+            graphed[d] = graphs.GraphedStep3D(nb, chi, d, policy(args.fp64_steps + 1), lnf_g, counter)
+        log(f"captured {len(graphed)} steady-step graphs")
+
+    def run_timed():
+        if not graphed:
+            return run(sched, policy, T0)
+        ops.fill_(lnf_g, 0.0)
+        T = ops.contiguous(T0, policy(0).store)
+        hotrg.normalize_trace(T, lnf_g, 0)
+        for n, d in sched:
+            if n <= args.fp64_steps:
+                T = hotrg.step3d(T, chi, d, policy(n))
+                hotrg.normalize_trace(T, lnf_g, n)
+            else:
+                if n == args.fp64_steps + 1:
+                    T = ops.cast(T, policy(n).store)
+                    graphs.set_step(counter, n)
+                T = graphed[d](T)
+        return T, lnf_g
+
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    with rec:
-        T, lnf = run(sched, policy, T0)
-        z = hotrg.lnz3d(T, lnf, K)
-        torch.cuda.synchronize()
+    T, lnf = run_timed()
+    z = hotrg.lnz3d(T, lnf, K)
+    torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     zt = z.cpu().numpy()
-    log(f"timed {K} steps x {nb} instances in {dt:.2f} s")
+    log(f"timed {K} steps x {nb} instances in {dt:.2f} s ({'graph replays' if graphed else 'eager'})")
+    # the chi^11 GEMM's launch time, from one more eager steady step (graph replays carry no
+    # per-launch events)
+    rec = ops.GemmRecorder("merge3d_x3")
+    with rec:
+        n_last, d_last = sched[-1]
+        hotrg.step3d(ops.cast(T, policy(n_last).store), chi, d_last, policy(n_last))
+        torch.cuda.synchronize()
     del T
     torch.cuda.empty_cache()
     extra = {"lnZ_per_site": {"betas": betas[:3], "values": zt[:3].tolist()}}
@@ -184,7 +217,8 @@ def bench_atrg3d(args):
         s, c = atrg.normalize(U, s, VH)
         consts = [c]
         for n in range(1, nsteps + 1):
-            U, s, VH = atrg.atrg_pure_tensor(U, s, VH, chi, "TXY"[(n - 1) % 3], prec)
+            U, s, VH = atrg.atrg_pure_tensor(U, s, VH, chi, "TXY"[(n - 1) % 3],
+                                             "f64" if n <= args.fp64_steps else prec)
             s, c = atrg.normalize(U, s, VH)
             consts.append(c)
         tr = atrg.trace(U, s, VH)
@@ -207,13 +241,17 @@ def bench_atrg3d(args):
         extra = {"lnZ_per_site": {"beta": 0.55, "value": z}}
         if not args.no_validate:
             t1 = time.perf_counter()
+            f64s, args.fp64_steps = args.fp64_steps, 0
             zd = solve(K, "f64", args.atrg_q)
+            args.fp64_steps = f64s
             extra["lnZ_err_vs_fp64"] = abs(z - zd)
             extra["fp64_same_q"] = {"value": zd, "steps_per_s": K / (time.perf_counter() - t1)}
     finally:
         atrg.RSVD.update(saved)
     cfg = {"workload": f"3D q=3 Potts ATRG chi={chi}, beta=0.55, {K} RG steps (T,X,Y), rSVD k=chi p=chi "
-                       f"q={args.atrg_q} seed 1", "chi": chi, "rg_steps": K, "precision": args.atrg_prec,
+                       f"q={args.atrg_q} seed 1", "chi": chi, "rg_steps": K,
+           "precision": f"f64 for steps <= {args.fp64_steps}, then {args.atrg_prec} contractions "
+                        "(decompositions and squeezer environments f64)",
            "parallelism": "single GPU, one instance"}
     print(json.dumps(_line(cfg, K / dt, K, args.warmup, dt, args.atrg_prec, None, extra)), flush=True)
     return 0
@@ -315,7 +353,8 @@ def main():
     args.chi = args.chi or {"hotrg2d": 64, "hotrg3d": 16, "atrg3d": 64}[wl]
     args.nbeta = args.nbeta or (1 if wl == "atrg3d" else 32)
     args.steps = args.steps or {"hotrg2d": 40, "hotrg3d": 6, "atrg3d": 30}[wl]
-    args.fp64_steps = args.fp64_steps if args.fp64_steps is not None else {"hotrg2d": 6, "hotrg3d": 4}.get(wl, 0)
+    args.fp64_steps = (args.fp64_steps if args.fp64_steps is not None
+                       else {"hotrg2d": 6, "hotrg3d": 4, "atrg3d": 9}.get(wl, 0))
     if args.traffic_probe:
         return traffic_probe(args)
     if wl == "hotrg3d":

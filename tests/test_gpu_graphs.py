@@ -76,3 +76,30 @@ def test_graph_replay_not_slower_than_eager(gpu):
     tg = (time.perf_counter() - t) / 10
     print(f"\n[graph] chi=16 B=1 step: eager {te * 1e3:.2f} ms, graph {tg * 1e3:.2f} ms ({te / tg:.1f}x)", flush=True)
     assert tg < 1.25 * te
+
+
+def test_graph_replays_equal_eager_steps_3d(gpu):
+    """Captured steady-state batched 3D HOTRG steps (Z, Y, X) replay bit for bit like eager steps."""
+    from trg_backup_amd import graphs, hotrg, ops
+    B, chi = 2, 6
+    T = torch.stack([torch.as_tensor(models.ising3d_tensor(b)) for b in (0.21, 0.23)]).to(gpu)
+    lnf = ops.fill_(torch.empty((B, 16), dtype=torch.float64, device=gpu), 0.0)
+    T = ops.contiguous(T)
+    hotrg.normalize_trace(T, lnf, 0)
+    for n, d in enumerate("ZYX", 1):                    # grow to the steady shape in float64
+        T = hotrg.step3d(T, chi, d, hotrg.FP64)
+        hotrg.normalize_trace(T, lnf, n)
+    T = ops.cast(T, torch.float32)
+    lnf_e, lnf_g = ops.contiguous(lnf), ops.contiguous(lnf)
+    counter = graphs.step_counter(gpu)
+    g = {d: graphs.GraphedStep3D(B, chi, d, hotrg.FP32, lnf_g, counter) for d in "ZYX"}
+    graphs.set_step(counter, 4)
+    Te, Tg = ops.contiguous(T), ops.contiguous(T)
+    for n in range(4, 10):
+        d = "ZYX"[(n - 1) % 3]
+        Te = hotrg.step3d(Te, chi, d, hotrg.FP32)
+        hotrg.normalize_trace(Te, lnf_e, n)
+        Tg = ops.contiguous(g[d](Tg))                   # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(Te, Tg), f"replay of step {n} differs"
+        assert torch.equal(lnf_e[:, n], lnf_g[:, n])

@@ -12,6 +12,9 @@ graph itself advances (trg_normalize with step_ptr).
     gY = GraphedStep2D(batch, chi, "Y", prec, lnf, step)
     gX = GraphedStep2D(batch, chi, "X", prec, lnf, step)
     set_step(step, n); T = gY(T); T = gX(T); ...
+
+GraphedStep3D does the same for steady-state 3D HOTRG steps (trace
+normalisation, trg/HOTRG_3d.py:232-238).
 """
 import torch
 
@@ -32,7 +35,7 @@ class GraphedStep2D:
     def __init__(self, batch, chi, direction, prec, lnf, step, warmup=1):
         dev = lnf.device
         self.batch, self.chi, self.direction, self.prec = batch, chi, direction, prec
-        self.inp = torch.empty((batch, chi, chi, chi, chi), dtype=prec.store, device=dev)
+        self.inp = torch.empty((batch,) + (chi,) * getattr(self, "ndim", 4), dtype=prec.store, device=dev)
         ops.fill_(self.inp, 1.0)
         # warm-up on a side stream writes into scratch log-norm / counter buffers
         scratch_lnf = ops.fill_(torch.empty(lnf.shape, dtype=torch.float64, device=dev), 0.0)
@@ -60,3 +63,17 @@ class GraphedStep2D:
             ops.copy_into(self.inp, T)
         self.graph.replay()
         return self.out
+
+
+class GraphedStep3D(GraphedStep2D):
+    """A captured steady-state batched 3D HOTRG step (legs chi^6)."""
+
+    def __init__(self, batch, chi, direction, prec, lnf, step, warmup=1):
+        self.ndim = 6
+        super().__init__(batch, chi, direction, prec, lnf, step, warmup)
+
+    def _body(self, lnf, step):
+        T = hotrg.step3d(self.inp, self.chi, self.direction, self.prec)
+        c = ops.trace_pairs(T, batch=self.batch)
+        ops.normalize_(T, c, lnf, 0, batch=self.batch, lnf_stride=lnf.shape[1], step_ptr=step)
+        return T

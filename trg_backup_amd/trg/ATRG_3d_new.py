@@ -21,11 +21,20 @@ from .._io import from_device, to_device
 atrg.degeneracy_eps = float(os.environ.get("DGENERACY_EPS", "0.0"))
 degeneracy_eps = atrg.degeneracy_eps
 truncate_eps = atrg.truncate_eps
-PRECISION = {"contract": "f64"}
+PRECISION = {"contract": "f64", "fp64_steps": 0}
 
 
-def set_precision(contract_dtype="f64"):
+def set_precision(contract_dtype="f64", fp64_steps=0):
+    """Contraction dtype of the coarse-graining GEMMs ("f64" or "f32"); the
+    first `fp64_steps` RG steps run in float64 regardless (an error made at
+    step n enters ln Z/V with weight 2^-n).  Decompositions, environments of
+    the squeezer and the normalisation are always float64."""
     PRECISION["contract"] = contract_dtype
+    PRECISION["fp64_steps"] = int(fp64_steps)
+
+
+def _step_precision(n):
+    return "f64" if n <= PRECISION["fp64_steps"] else PRECISION["contract"]
 
 
 def set_rsvd_params(k_factor=3, p=0, q=None, seed=12345):
@@ -134,7 +143,7 @@ def pure_tensor(T, Dcut, TOT_RGSTEPS):
             break
         if rgstep[direction] < TOT_RGSTEPS[direction]:
             rgstep[direction] += 1
-            U, s, VH = atrg.atrg_pure_tensor(U, s, VH, Dcut, direction, PRECISION["contract"])
+            U, s, VH = atrg.atrg_pure_tensor(U, s, VH, Dcut, direction, _step_precision(sum(rgstep.values())))
             s, c = atrg.normalize(U, s, VH)
             key = f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
             T.normal_const[key] = math.log(abs(float(c[0].item()))) / 2 ** (sum(rgstep.values()))
