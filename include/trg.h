@@ -224,13 +224,17 @@ int trg_potrf_psd_f64(int batch, int l, double *G, long long sG, double psd_tol,
 
 /*
  * Orthonormal basis of the column span of batch m x l float64 matrices
- * (l <= 256 <= ... l <= m) by shifted CholeskyQR3; X (stride sX) is
- * clobbered, Q is written with stride m*l.  Replaces cupy.linalg.qr in the
- * range finders (trg/ATRG_3d_new.py:104,111,117, utility/randomized_svd.py:59,84-91).
+ * (l <= 256, l <= m) by shifted CholeskyQR3 (passes = 3), or a
+ * well-conditioned basis of the same span from its shifted first pass
+ * (passes = 1, for the inner steps of a power iteration); X (stride sX) is
+ * clobbered, Q is written with stride m*l.  Tall blocks with few output
+ * tiles split the Gram over row slabs (count fixed by m and l) summed in
+ * slab order.  Replaces cupy.linalg.qr in the range finders
+ * (trg/ATRG_3d_new.py:104,111,117, utility/randomized_svd.py:59,84-91).
  */
 long long trg_orthonormalize_workspace(int batch, int m, int l);
 int trg_orthonormalize_f64(int batch, int m, int l, double *X, long long sX, double *Q, void *ws,
-                           long long ws_bytes, int *info, void *stream);
+                           long long ws_bytes, int passes, int *info, void *stream);
 
 /* Y[b][i] += alpha_rel * scal[b] * X[b][i] (float64, device-side scale). */
 int trg_axpy_dev_f64(int batch, long long n, double alpha_rel, const double *scal, const double *X,

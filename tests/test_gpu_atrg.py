@@ -189,3 +189,18 @@ def test_atrg_few_power_iterations_within_rsvd_accuracy(gpu, q):
         oat.degeneracy_eps, atrg.degeneracy_eps = old
     print(f"\n[atrg potts q={q}] gpu-oracle={v - ref:.3e} oracle(q)-oracle(converged)={ref - conv:.3e}", flush=True)
     assert abs(v - ref) <= 10 * abs(ref - conv) + 1e-12
+
+
+@pytest.mark.parametrize("m,l,passes", [(32768, 64, 3), (32768, 64, 1), (4096, 128, 3), (300, 40, 3)])
+def test_orthonormalize_tall_blocks(gpu, m, l, passes):
+    """Tall blocks take the split (row-slab) Gram; three passes give an orthonormal basis,
+    one pass a basis of the same span.  Both reproduce X = Q (Q^+ X)."""
+    from trg_backup_amd import linalg
+    rng = np.random.default_rng(m + l)
+    X = rng.standard_normal((1, m, l)) @ np.diag(np.logspace(0, -6, l))          # graded columns
+    Q = linalg.orthonormalize(torch.as_tensor(X).to(gpu), passes=passes)[0].cpu().numpy()
+    G = Q.T @ Q
+    if passes == 3:
+        assert np.abs(G - np.eye(l)).max() < 1e-12
+    coef = np.linalg.lstsq(Q, X[0], rcond=None)[0]
+    assert np.abs(Q @ coef - X[0]).max() < 1e-9 * np.abs(X).max()

@@ -372,17 +372,64 @@ extern "C" long long trg_eigh_topk_workspace(int batch, int n, int l) {
     return dbl * (long long)sizeof(double) + 4LL * batch + 256;
 }
 
+constexpr int GRAM_SLABS_MAX = 64;
+
+// out = sum of `count` consecutive slabs of `len` doubles (fixed order).
+__global__ void sum_slabs_kernel(int count, long long len, const double *__restrict__ P, double *__restrict__ out) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len; e += (long long)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int j = 0; j < count; ++j) acc += P[j * len + e];
+        out[e] = acc;
+    }
+}
+
+// Slab count of the split Gram of an n x l block: a function of the shape only,
+// so an instance's summation order never depends on the batch it runs in.
+static int gram_slabs(int n, int l) {
+    const long long tiles = cdiv(l, 64) * cdiv(l, 64);
+    if (tiles >= 16) return 1;
+    return (int)std::max<long long>(1, std::min<long long>(GRAM_SLABS_MAX, n / 256));
+}
+
+// G[b] = X[b]^T X[b] for a tall-skinny n x l block.  The l x l product has
+// only a handful of output tiles, so with `part` (GRAM_SLABS_MAX * l * l
+// doubles per instance) the n rows are split into slabs whose partial Grams
+// run as one batched GEMM and are summed in slab order.
+static int gram_f64(int batch, int n, int l, const double *X, long long sX, double *G, double *part,
+                    hipStream_t s) {
+    const long long sG = (long long)l * l;
+    const int S = part ? gram_slabs(n, l) : 1;
+    if (S < 2) return trg_gemm(TRG_F64, TRG_F64, batch, l, l, n, 1.0, X, sX, 1, l, X, sX, l, 1, 0.0, G, sG, l, 1, s);
+    const int kc = n / S;
+    const int kl = n - (S - 1) * kc;
+    for (int b = 0; b < batch; ++b) {
+        const double *Xb = X + (long long)b * sX;
+        double *Pb = part + (long long)b * S * sG;
+        TRG_TRY(trg_gemm(TRG_F64, TRG_F64, S - 1, l, l, kc, 1.0, Xb, (long long)kc * l, 1, l, Xb, (long long)kc * l,
+                         l, 1, 0.0, Pb, sG, l, 1, s));
+        const double *Xt = Xb + (long long)(S - 1) * kc * l;
+        TRG_TRY(trg_gemm(TRG_F64, TRG_F64, 1, l, l, kl, 1.0, Xt, 0, 1, l, Xt, 0, l, 1, 0.0, Pb + (S - 1) * sG, 0, l, 1,
+                         s));
+        hipLaunchKernelGGL(sum_slabs_kernel, (unsigned)std::min<long long>(cdiv(sG, 256), 1024), 256, 0, s, S, sG, Pb,
+                           G + (long long)b * sG);
+        TRG_TRY(check_launch());
+    }
+    return TRG_OK;
+}
+
 // Orthonormalise X (batch of n x l) with shifted CholeskyQR3 (one shifted
-// pass, two plain passes; Fukaya et al., SIAM J. Sci. Comput. 42 (2020)).
-// After the three passes the orthonormal result is in Y (X is clobbered).
+// pass, two plain passes; Fukaya et al., SIAM J. Sci. Comput. 42 (2020)), or
+// with the shifted pass alone when passes == 1 (a well-conditioned basis of
+// the same span, enough inside a power iteration).  The result lands in Y
+// (X is clobbered).
 static int cholqr3(int batch, int n, int l, double *X, double *Y, double *G, double *R, int *flags, int *info,
-                   hipStream_t s) {
+                   hipStream_t s, int passes = 3, double *part = nullptr) {
     const double shift0 = 11.0 * ((double)n * l + (double)l * (l + 1)) * 1.1102230246251565e-16;
     const long long sX = (long long)n * l, sG = (long long)l * l;
     double *src = X, *dst = Y;
-    for (int pass = 0; pass < 3; ++pass) {
+    for (int pass = 0; pass < passes; ++pass) {
         // G = src^T src
-        TRG_TRY(trg_gemm(TRG_F64, TRG_F64, batch, l, l, n, 1.0, src, sX, 1, l, src, sX, l, 1, 0.0, G, sG, l, 1, s));
+        TRG_TRY(gram_f64(batch, n, l, src, sX, G, part, s));
         TRG_TRY(trg_chol_inv_f64(batch, l, G, sG, pass == 0 ? shift0 : 0.0, R, flags, s));
         if (info) {
             hipLaunchKernelGGL(or_info_kernel, (unsigned)cdiv(batch, 256), 256, 0, s, batch, flags, info);
@@ -662,14 +709,17 @@ extern "C" int trg_potrf_psd_f64(int batch, int l, double *G, long long sG, doub
 
 extern "C" long long trg_orthonormalize_workspace(int batch, int m, int l) {
     if (batch <= 0 || m <= 0 || l <= 0) return 0;
-    return (long long)batch * ((long long)m * l + 2LL * l * l) * (long long)sizeof(double) + 4LL * batch + 256;
+    const long long slabs = gram_slabs(m, l) > 1 ? (long long)gram_slabs(m, l) * l * l : 0;
+    return (long long)batch * ((long long)m * l + 2LL * l * l + slabs) * (long long)sizeof(double) + 4LL * batch +
+           256;
 }
 
 extern "C" int trg_orthonormalize_f64(int batch, int m, int l, double *X, long long sX, double *Q, void *ws,
-                                      long long ws_bytes, int *info, void *stream) {
+                                      long long ws_bytes, int passes, int *info, void *stream) {
     // X (batch x m x l, stride sX) is clobbered; the orthonormal basis of its
-    // column span lands in Q (stride m*l).  Shifted CholeskyQR3.
-    if (batch < 0 || m < 0 || l < 0) return TRG_BAD_ARG;
+    // column span lands in Q (stride m*l).  Shifted CholeskyQR3 (passes = 3)
+    // or its shifted first pass alone (passes = 1).
+    if (batch < 0 || m < 0 || l < 0 || (passes != 1 && passes != 3)) return TRG_BAD_ARG;
     if (batch == 0 || m == 0 || l == 0) return TRG_OK;
     if (l > 256 || l > m) return TRG_SHAPE;
     if (!X || !Q || !ws) return TRG_BAD_ARG;
@@ -679,12 +729,14 @@ extern "C" int trg_orthonormalize_f64(int batch, int m, int l, double *X, long l
     double *Xc = static_cast<double *>(ws);
     double *G = Xc + batch * mx;
     double *R = G + batch * sG;
-    int *flags = reinterpret_cast<int *>(R + batch * sG);
+    double *part = R + batch * sG;
+    const long long slabs = gram_slabs(m, l) > 1 ? (long long)gram_slabs(m, l) * sG : 0;
+    int *flags = reinterpret_cast<int *>(part + batch * slabs);
     long long dims[2] = {batch, mx}, si[2] = {sX, 1}, so[2] = {mx, 1};
     TRG_TRY(trg_copy_strided(TRG_F64, TRG_F64, 2, dims, X, si, Xc, so, s));
-    if (info) TRG_TRY(trg_fill(3, batch, 0.0, info, s));
-    // passes: Xc -> Q -> Xc -> Q
-    TRG_TRY(cholqr3(batch, m, l, Xc, Q, G, R, flags, info, s));
+    if (info) TRG_TRY(trg_fill(TRG_I32, batch, 0.0, info, s));
+    // odd pass counts: Xc -> Q (-> Xc -> Q)
+    TRG_TRY(cholqr3(batch, m, l, Xc, Q, G, R, flags, info, s, passes, slabs ? part : nullptr));
     return TRG_OK;
 }
 

@@ -46,10 +46,13 @@ def _fits_small(m, n):
     return n <= m <= 256 and m * n * 8 + 16384 <= 163840
 
 
-def orthonormalize(X, perturb=PERTURB, stream_id=7):
+def orthonormalize(X, perturb=PERTURB, stream_id=7, passes=3):
     """Orthonormal basis (B, m, l) of the columns of X (B, m, l), l <= m.
     A relative Gaussian perturbation keeps rank-deficient blocks of full
-    column rank (directions below perturb * max|X| are noise either way)."""
+    column rank (directions below perturb * max|X| are noise either way).
+    passes=1: one shifted CholeskyQR pass, a well-conditioned (not fully
+    orthonormal) basis of the same span, for the inner steps of a power
+    iteration."""
     X = ops.contiguous(_f64(X))
     B, m, l = X.shape
     if perturb > 0:
@@ -62,7 +65,8 @@ def orthonormalize(X, perturb=PERTURB, stream_id=7):
     ws = torch.empty(wsb // 8 + 1, dtype=torch.float64, device=X.device)
     Q = torch.empty((B, m, l), dtype=torch.float64, device=X.device)
     info = torch.empty(B, dtype=torch.int32, device=X.device)
-    call("trg_orthonormalize_f64", B, m, l, ptr(X), m * l, ptr(Q), ptr(ws), ws.numel() * 8, ptr(info), stream_ptr())
+    call("trg_orthonormalize_f64", B, m, l, ptr(X), m * l, ptr(Q), ptr(ws), ws.numel() * 8, passes, ptr(info),
+         stream_ptr())
     return Q
 
 
@@ -96,10 +100,14 @@ def svd_topk(A, l, q, seed=RSVD_SEED, Aop=None, ATop=None, shape=None):
     else:
         B, m, n = shape
     Om = ops.gaussian((n, l), seed=seed, device="cuda", batch=B, stream_stride=0).view(B, n, l)
-    Q = orthonormalize(Aop(Om))
-    for _ in range(q):
-        Z = orthonormalize(ATop(Q))
-        Q = orthonormalize(Aop(Z))
+    # inner iterates only need a well-conditioned basis of their span (one
+    # shifted CholeskyQR pass); the basis the result is built from is the
+    # fully orthonormalised last one
+    inner = dict(perturb=0.0, passes=1)
+    Q = orthonormalize(Aop(Om), **(inner if q > 0 else {}))
+    for it in range(q):
+        Z = orthonormalize(ATop(Q), **inner)
+        Q = orthonormalize(Aop(Z), **(inner if it < q - 1 else {}))
     Bt = ATop(Q)                                   # (B, n, l) = (Q^T A)^T
     v, s, uh = svd_thin(Bt)                        # Bt = v s uh  ->  Q^T A = uh^T s v^T
     return gemm(Q, uh.transpose(1, 2)), s, v.transpose(1, 2)
