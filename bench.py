@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2, help="warm-up steps per precision tier")
     ap.add_argument("--chi", type=int, default=None, help="hotrg2d 64, hotrg3d 16, atrg3d 64")
     ap.add_argument("--nbeta", type=int, default=None, help="hotrg2d 32, hotrg3d 32 (of the 128-value draw)")
-    ap.add_argument("--fp64-steps", type=int, default=None, help="hotrg2d 6, hotrg3d 4, atrg3d 12")
+    ap.add_argument("--fp64-steps", type=int, default=None, help="hotrg2d 6, hotrg3d 4, atrg3d 16")
     ap.add_argument("--atrg-q", type=int, default=2)
     ap.add_argument("--atrg-prec", default="f32", choices=["f32", "f64"])
     ap.add_argument("--split-merge-from", type=int, default=10, help="bf16x3 merge after this step (-1: never)")
@@ -354,7 +354,7 @@ def main():
     args.nbeta = args.nbeta or (1 if wl == "atrg3d" else 32)
     args.steps = args.steps or {"hotrg2d": 40, "hotrg3d": 6, "atrg3d": 30}[wl]
     args.fp64_steps = (args.fp64_steps if args.fp64_steps is not None
-                       else {"hotrg2d": 6, "hotrg3d": 4, "atrg3d": 12}.get(wl, 0))
+                       else {"hotrg2d": 6, "hotrg3d": 4, "atrg3d": 16}.get(wl, 0))
     if args.traffic_probe:
         return traffic_probe(args)
     if wl == "hotrg3d":
