@@ -48,9 +48,11 @@ MFMA_PER_PRODUCT = {"f32": 1, "bf16": 1, "bf16x3": 3, "f64": 1}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="hotrg2d", choices=["hotrg2d", "hotrg3d", "atrg3d", "serve"],
-                    help="hotrg2d: the headline line; hotrg3d / atrg3d: the batched 3D Ising HOTRG and the "
-                         "3D q=3 Potts ATRG configurations; serve: a Poisson request stream (secondary lines)")
+    ap.add_argument("--workload", default="hotrg2d",
+                    choices=["hotrg2d", "hotrg3d", "hotrg3d_qr", "atrg3d", "serve"],
+                    help="hotrg2d: the headline line; hotrg3d / hotrg3d_qr / atrg3d: the batched 3D Ising HOTRG "
+                         "(trg/HOTRG_3d.py, trg/HOTRG_3d_QR.py) and the 3D q=3 Potts ATRG configurations; serve: "
+                         "a Poisson request stream (secondary lines)")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed RG steps of the schedule (hotrg2d 40 = whole solve; hotrg3d 6; atrg3d 30)")
     ap.add_argument("--warmup", type=int, default=2, help="warm-up steps per precision tier")
@@ -58,6 +60,7 @@ def parse():
     ap.add_argument("--nbeta", type=int, default=None, help="hotrg2d 32, hotrg3d 32 (of the 128-value draw)")
     ap.add_argument("--fp64-steps", type=int, default=None, help="hotrg2d 6, hotrg3d 4, atrg3d 16")
     ap.add_argument("--atrg-q", type=int, default=2)
+    ap.add_argument("--squeezer", default="QR", choices=["original", "QR", "fix"], help="hotrg3d_qr squeezer")
     ap.add_argument("--atrg-prec", default="f32", choices=["f32", "f64"])
     ap.add_argument("--split-merge-from", type=int, default=10, help="bf16x3 merge after this step (-1: never)")
     ap.add_argument("--bf16-env-from", type=int, default=12, help="bf16 environments after this step (-1: never)")
@@ -193,6 +196,55 @@ def bench_hotrg3d(args):
                        "from default_rng(0).uniform(0.20,0.24) batched",
            "chi": chi, "nbeta": nb, "rg_steps": K, "policy": policy.describe(), "parallelism": "single GPU, batched"}
     roof = _roof(rec, "merge chi^11 GEMM (T.U1.U2 x T, f32 MFMA)", PEAK_TFLOPS["f32"])
+    print(json.dumps(_line(cfg, nb * K / dt, K, args.warmup, dt, "f32", roof, extra)), flush=True)
+    return 0
+
+
+def bench_hotrg3d_qr(args):
+    """Batched 3D Ising HOTRG with squeezers (trg/HOTRG_3d_QR.py, the 3D HOTRG the reference's 3D
+    drivers run): beta from numpy.random.default_rng(0).uniform(0.20, 0.24, 128) (the first --nbeta),
+    chi = 16, steps in the "ZXY" cycle from the initial tensor, squeezer --squeezer; float64 for the
+    first --fp64-steps steps, then float32 MFMA merges with float64 environments and projectors."""
+    import numpy as np
+    import torch
+
+    from trg_backup_amd import _lib, autotune, hotrg, hotrg_qr, models, ops
+    _lib.load()
+    dev = torch.device("cuda:0")
+    chi, nb, K = args.chi, args.nbeta, args.steps
+    betas = np.random.default_rng(0).uniform(0.20, 0.24, 128)[:nb].tolist()
+    policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32)
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    T0 = torch.stack([models.initial_tensor("ising3d", b, dev) for b in betas])
+    order = hotrg_qr.schedule({"X": K, "Y": K, "Z": K})[:K]
+    tot = {d: order.count(d) for d in "XYZ"}
+
+    autotune.enable(not args.no_tune)
+    t0 = time.perf_counter()
+    warm = hotrg_qr.schedule({"X": K, "Y": K, "Z": K})[:min(K, args.fp64_steps + 2)]
+    hotrg_qr.renorm(T0, chi, {d: warm.count(d) for d in "XYZ"}, args.squeezer, 0.0, policy)
+    torch.cuda.synchronize()
+    autotune.enable(False)
+    log(f"warm-up in {time.perf_counter() - t0:.1f} s")
+    torch.cuda.empty_cache()
+    rec = ops.GemmRecorder("merge3d_x3")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with rec:
+        z = hotrg_qr.ln_z_over_v(T0, chi, tot, args.squeezer, 0.0, policy)
+        torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    zt = z.cpu().numpy()
+    log(f"timed {K} steps x {nb} instances in {dt:.2f} s")
+    extra = {"lnZ_per_site": {"betas": betas[:3], "values": zt[:3].tolist()}, "squeezer": args.squeezer}
+    if not args.no_validate:
+        sel = [0, nb - 1] if nb > 1 else [0]
+        zd = hotrg_qr.ln_z_over_v(ops.contiguous(T0[sel]), chi, tot, args.squeezer, 0.0).cpu().numpy()
+        extra["lnZ_err_vs_fp64"] = float(max(abs(zt[i] - zd[j]) for j, i in enumerate(sel)))
+    cfg = {"workload": f"3D Ising HOTRG_3d_QR ({args.squeezer} squeezer) chi={chi}, {K} RG steps (ZXY cycle) "
+                       f"from the initial tensor, {nb} beta from default_rng(0).uniform(0.20,0.24) batched",
+           "chi": chi, "nbeta": nb, "rg_steps": K, "policy": policy.describe(), "parallelism": "single GPU, batched"}
+    roof = _roof(rec, "merge chi^11 GEMM (T.Px.Py x T, f32 MFMA)", PEAK_TFLOPS["f32"])
     print(json.dumps(_line(cfg, nb * K / dt, K, args.warmup, dt, "f32", roof, extra)), flush=True)
     return 0
 
@@ -350,15 +402,17 @@ def main():
                                    "(independent couplings shard trivially across GPUs, see DESIGN.md)"}))
         return 2
     wl = args.workload
-    args.chi = args.chi or {"hotrg2d": 64, "hotrg3d": 16, "atrg3d": 64}[wl]
+    args.chi = args.chi or {"hotrg2d": 64, "hotrg3d": 16, "hotrg3d_qr": 16, "atrg3d": 64}.get(wl, 64)
     args.nbeta = args.nbeta or (1 if wl == "atrg3d" else 32)
-    args.steps = args.steps or {"hotrg2d": 40, "hotrg3d": 6, "atrg3d": 30}[wl]
+    args.steps = args.steps or {"hotrg2d": 40, "hotrg3d": 6, "hotrg3d_qr": 6, "atrg3d": 30}.get(wl, 40)
     args.fp64_steps = (args.fp64_steps if args.fp64_steps is not None
-                       else {"hotrg2d": 6, "hotrg3d": 4, "atrg3d": 16}.get(wl, 0))
+                       else {"hotrg2d": 6, "hotrg3d": 4, "hotrg3d_qr": 4, "atrg3d": 16}.get(wl, 0))
     if args.traffic_probe:
         return traffic_probe(args)
     if wl == "hotrg3d":
         return bench_hotrg3d(args)
+    if wl == "hotrg3d_qr":
+        return bench_hotrg3d_qr(args)
     if wl == "atrg3d":
         return bench_atrg3d(args)
     if wl == "serve":

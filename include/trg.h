@@ -161,7 +161,7 @@ int trg_gaussian(int dtype, int batch, long long n, unsigned long long seed, uns
 /*
  * Top-k eigenpairs of batch symmetric PSD n x n float64 matrices by
  * block subspace iteration (block l, q iterations, shift-regularised
- * CholeskyQR) and Rayleigh-Ritz with trg_syevj_f64.  Outputs the k
+ * CholeskyQR3) and Rayleigh-Ritz with trg_syevj_f64.  Outputs the k
  * largest eigenvalues ascending in w (stride k), their eigenvectors as the
  * columns of V (row-major n x k, stride n*k), and the discarded mass
  * eps[b] = tr(A_b) - sum(w_b).  Needs ws_bytes >= trg_eigh_topk_workspace.
@@ -184,7 +184,8 @@ int trg_select_isometry(int dtype, int batch, long long nk, const double *eps0, 
 
 /*
  * Diagonal scaling of one leg:  Y_b[i] *= f(x_b[(i / inner) % period]) with
- * f = x, sqrt(x), 1/sqrt(x), 1/x for mode 0..3 (f32 or f64).  Replaces the
+ * f = x, sqrt(max(x, 0)), 1/sqrt(x), 1/x for mode 0..3 (f32 or f64; mode 1
+ * maps eigenvalues a rounding below zero to 0, NaN stays NaN).  Replaces the
  * s_i leg products of trg/ATRG_3d_new.py:180,188,278-279,285-286,451-452 and
  * tensor_class/tensor_class.py:66-86.
  */

@@ -204,3 +204,24 @@ def test_orthonormalize_tall_blocks(gpu, m, l, passes):
         assert np.abs(G - np.eye(l)).max() < 1e-12
     coef = np.linalg.lstsq(Q, X[0], rcond=None)[0]
     assert np.abs(Q @ coef - X[0]).max() < 1e-9 * np.abs(X).max()
+
+
+@pytest.mark.parametrize("case", ["decaying", "rank_deficient"])
+def test_truncate_known_matches_full_spectrum(gpu, case):
+    """The rank rule over the leading values plus the discarded mass equals
+    the rule over the whole spectrum (oracle truncate), including an exactly
+    rank-deficient spectrum whose computed tail is rounding noise."""
+    from trg_backup_amd import linalg
+    if case == "decaying":
+        full = 10.0 ** (-0.8 * np.arange(40))            # tail rule cuts at 13 < k
+        k, deg, known = 16, 0.0, 18
+        tail_sq = np.sum(full[known:] ** 2)
+    else:
+        full = np.concatenate([np.array([1.0, 0.7, 0.7, 0.3, 0.1, 0.02]), np.zeros(30)])
+        k, deg, known = 10, 1e-8, 12
+        tail_sq = 3e-17                                   # what ||P||^2 - sum s^2 leaves behind
+    total = np.sum(full ** 2)
+    ref = otsvd.truncate(full, k, deg, 1e-10)
+    chi, _ = linalg.truncate_known(_dev(full[:known], gpu), _dev(np.array([tail_sq]), gpu),
+                                   _dev(np.array([total]), gpu), k, deg, 1e-10)
+    assert chi == ref[0], (chi, ref)

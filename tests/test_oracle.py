@@ -187,3 +187,80 @@ def test_leg_swap_rsvd_matches_dense_svd():
     UM, sM, VMH = at.rsvd_for_ATRG_leg_swapping(B, C, chi_k=9, p=0, q=2, eps=1e-12, seed=7)
     assert np.allclose(sM, np.linalg.svd(M, compute_uv=False)[:len(sM)])
     assert np.allclose(np.einsum("igk,k,kjb->igjb", UM, sM, VMH).reshape(9, 16), M)
+
+
+# ---------------------------------------------------------------------------
+# HOTRG 3D with squeezers (trg/HOTRG_3d_QR.py)
+# ---------------------------------------------------------------------------
+from oracle import HOTRG_3d_QR as hq  # noqa: E402
+
+
+@pytest.mark.parametrize("kind", ["original", "QR", "fix"])
+def test_hotrg3d_qr_untruncated_equals_brute_force(kind):
+    """Z, X, Y on a 2x2x2 torus with Dcut = 16: no truncation, so every
+    squeezer must give the exact partition function (the oblique QR
+    projectors multiply to the identity on the kept space)."""
+    beta = 0.3
+    v, _ = hq.ln_Z_over_V(models.ising3d_tensor(beta), 16, {"X": 1, "Y": 1, "Z": 1}, kind)
+    assert abs(v - exact.brute_lnZ_ising3d(beta)) < 1e-12
+
+
+@pytest.mark.parametrize("steps,shape", [({"X": 0, "Y": 0, "Z": 1}, (1, 1, 2)),
+                                         ({"X": 1, "Y": 0, "Z": 1}, (2, 1, 2))])
+@pytest.mark.parametrize("kind", ["original", "QR"])
+def test_hotrg3d_qr_potts_boxes(kind, steps, shape):
+    v, _ = hq.ln_Z_over_V(models.potts3d_tensor(0.55), 81, steps, kind)
+    assert abs(v - exact.brute_lnZ_potts3d(0.55, shape=shape)) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["original", "QR", "fix"])
+def test_hotrg3d_qr_beta0_is_ln2(kind):
+    v, _ = hq.ln_Z_over_V(models.ising3d_tensor(0.0), 4, {"X": 2, "Y": 2, "Z": 2}, kind)
+    assert abs(v - np.log(2)) < 1e-13
+
+
+def test_hotrg3d_qr_environments_and_coarse_graining_match_einsum():
+    """The environments against the reference's four-operand strings
+    (trg/HOTRG_3d_QR.py:212-215) and the pairwise coarse graining against the
+    one-shot einsum of :388, on random tensors with unequal bonds."""
+    rng = np.random.default_rng(3)
+    T0 = rng.standard_normal((2, 3, 2, 3, 2, 3))
+    T1 = rng.standard_normal((2, 3, 2, 3, 3, 2))
+    strings = {1: "aibcdm,ejfgmh,akbcdn,elfgnh->ijkl", 0: "iabcdm,jefgmh,kabcdn,lefgnh->ijkl",
+               2: "abicdm,efjgmh,abkcdn,eflgnh->ijkl", 3: "abcidm,efgjmh,abckdn,efglnh->ijkl"}
+    for leg, s in strings.items():
+        ref = np.einsum(s, T0, T1, T0, T1)
+        n = T0.shape[leg] * T1.shape[leg]
+        assert np.allclose(hq.environment(T0, T1, leg), ref.reshape(n, n))
+    Px = rng.standard_normal((4, 2, 2)); PX = rng.standard_normal((3, 3, 5))
+    Py = rng.standard_normal((2, 2, 3)); PY = rng.standard_normal((2, 3, 3))
+    assert np.allclose(hq.coarse_graining(T0, T1, Px, PX, Py, PY),
+                       hq.coarse_graining_einsum(T0, T1, Px, PX, Py, PY))
+
+
+def test_hotrg3d_qr_projector_identities():
+    """Px PX = 1 and PY Py = 1: the QR squeezer's projectors are biorthogonal
+    (S^-1/2 U^H (RL RR) V S^-1/2 = 1).  The 'original' isometry is
+    orthonormal and spans the top eigenvectors of the x-leg environment (the
+    reference's side comparison sums empty slices, so that side always wins)."""
+    T = hq.leg_transposition(models.ising3d_tensor(0.25), "transpose", "Z")
+    T = hq.new_pure_tensor(T, 3, "Z", "QR")               # a tensor with bonds 3, 3, 3, 3, 2, 2
+    Px, PX, Py, PY, _ = hq.squeezer(T, T, T, T, 4, "QR")
+    k = Px.shape[0]
+    assert np.allclose(Px.reshape(k, -1) @ PX.reshape(-1, k), np.eye(k), atol=1e-10)
+    ky = PY.shape[0]
+    assert np.allclose(PY.reshape(ky, -1) @ Py.reshape(-1, ky), np.eye(ky), atol=1e-10)
+    Px, PX, Py, PY, _ = hq.squeezer(T, T, T, T, 4, "original")
+    U = PX.reshape(-1, PX.shape[2])
+    assert np.allclose(U.T @ U, np.eye(U.shape[1]), atol=1e-12)
+    w, V = np.linalg.eigh(hq.environment(T, T, 0))
+    top = V[:, ::-1][:, :U.shape[1]]
+    assert np.allclose(U @ U.T, top @ top.T, atol=1e-10)
+
+
+def test_hotrg3d_qr_slice_legs():
+    for leg, sl in [(7, 3), (64, 5), (3, 8), (10, 1), (16, 16)]:
+        parts, n = hq.slice_legs(leg, sl)
+        assert n == len(parts) == min(leg, max(sl, 1))
+        covered = [i for p in parts for i in range(p.start, p.stop)]
+        assert covered == list(range(leg))

@@ -1,7 +1,7 @@
 """CPU checks of the drop-in surface: module aliases and the reference's public names.
 
 The name lists are the public functions of the reference modules on the path
-(trg/HOTRG_2d.py:8-300, trg/HOTRG_3d.py:18-398, trg/ATRG_3d_new.py:18-974,
+(trg/HOTRG_2d.py:8-300, trg/HOTRG_3d.py:18-398, trg/HOTRG_3d_QR.py:29-1033, trg/ATRG_3d_new.py:18-974,
 utility/truncated_svd.py:17-151, utility/randomized_svd.py:5-165,
 tensor_class/tensor_class.py:5-41).
 """
@@ -18,6 +18,12 @@ SURFACE = {
                      "new_pure_tensor", "new_impuer_tensor_2imp", "new_impuer_tensor_2to1imp",
                      "new_impuer_tensor_1imp", "__tensor_normalization__", "pure_tensor_renorm",
                      "two_point_func_renorm", "ynearest_two_point_func_renorm"],
+    "trg.HOTRG_3d_QR": ["leg_transposition", "squeezer_fix", "squeezer_original", "squeezer_QR", "slice_legs",
+                        "coarse_graining", "squeezer", "new_pure_tensor", "new_impuer_tensor_2points",
+                        "new_impuer_tensor_absorb", "new_impuer_tensor_1point", "cal_X", "find_1stto3rdmax",
+                        "cal_scaling_dimensions", "cal_coorelation", "normalization", "exec_rgstep_pure_tensor",
+                        "pure_tensor", "cal_field_expected_value", "exec_rgstep_1point_func", "one_point_function",
+                        "exec_rgstep_2point_func", "two_point_function"],
     "trg.ATRG_3d_new": ["leg_transposition", "rsvd_for_ATRG_leg_swapping", "leg_swapping", "squeezer",
                         "leg_slicing", "coarse_graining", "atrg_pure_tensor", "normalization",
                         "exec_pure_tensor_renorm", "pure_tensor"],
@@ -65,3 +71,18 @@ def test_numpy_input_without_library_or_gpu_fails_loudly(monkeypatch):
     with pytest.raises(Exception) as e:
         truncated_svd.svd(np.eye(4), shape=[[0], [1]], k=2)
     assert isinstance(e.value, (_lib.TrgError, RuntimeError, AssertionError))
+
+
+@pytest.mark.parametrize("leg,sl", [(7, 3), (64, 5), (3, 8), (10, 1), (16, 16), (100, 7)])
+def test_hotrg3d_qr_slice_legs_matches_oracle(leg, sl):
+    """The drop-in's host slicing equals the oracle restatement of trg/HOTRG_3d_QR.py:284-316."""
+    from oracle import HOTRG_3d_QR as oq
+    from trg_backup_amd.trg import HOTRG_3d_QR as dq
+    assert dq.slice_legs(leg, sl) == oq.slice_legs(leg, sl)
+
+
+def test_hotrg3d_qr_schedule_and_keys():
+    """"ZXY" cycle with per-direction counts (trg/HOTRG_3d_QR.py:780-786) and the "(X,Y,Z)" factor keys."""
+    from trg_backup_amd import hotrg_qr
+    assert hotrg_qr.schedule({"X": 2, "Y": 1, "Z": 3}) == ["Z", "X", "Y", "Z", "X", "Z"]
+    assert hotrg_qr.factor_keys({"X": 1, "Y": 1, "Z": 1}) == ["(0,0,0)", "(0,0,1)", "(1,0,1)", "(1,1,1)"]

@@ -167,8 +167,12 @@ def squeezer_environments(A, UM, sM, VMH, D):
 EIG_Q = 12   # power iterations of the top-k SVD of RX Rx when it is too large for the dense path
 
 
-def _oblique_pair(Env_big, Env_small, Dcut, shape_big, shape_small, dt):
-    """:278-292 with R^T R = Env_big (RX) and Rx Rx^T = Env_small; returns Px, PX."""
+def _oblique_pair(Env_big, Env_small, Dcut, shape_big, shape_small, dt, deg_eps=None, trunc_eps=None):
+    """:278-292 with R^T R = Env_big (RX) and Rx Rx^T = Env_small; returns Px, PX.
+    The rank rule uses this module's DGENERACY_EPS / truncate_eps unless
+    deg_eps / trunc_eps are given."""
+    deg_eps = degeneracy_eps if deg_eps is None else deg_eps
+    trunc_eps = truncate_eps if trunc_eps is None else trunc_eps
     RX = linalg.chol_psd_upper(Env_big[None])[0]                     # R^T R = XdagX
     Rx = ops.contiguous(linalg.chol_psd_upper(Env_small[None])[0].transpose(0, 1))   # Rx Rx^T = xxdag
     P = ops.gemm(RX, Rx)
@@ -177,12 +181,12 @@ def _oblique_pair(Env_big, Env_small, Dcut, shape_big, shape_small, dt):
     if linalg._fits_small(n, n):
         u, s, vh = linalg.svd_small(P[None])
         u, s, vh = u[0], s[0], vh[0]
-        chi, _ = linalg.truncate(s, k1, degeneracy_eps, truncate_eps)
+        chi, _ = linalg.truncate(s, k1, deg_eps, trunc_eps)
     else:
         l = min(n, max(2 * k1, k1 + 8), linalg.SMALL_MAX)
         u, s, vh = linalg.svd_topk(P[None], l, EIG_Q)
         u, s, vh = u[0], s[0], vh[0]
-        chi, _ = _truncate_with_tail(s, P, k1)
+        chi, _ = linalg.truncate_with_tail(s, P, k1, deg_eps, trunc_eps)
     chi = min(chi, Dcut)
     sis = ops.contiguous(s[:chi])
     v = ops.contiguous(vh[:chi].transpose(0, 1))                      # (n, chi)
@@ -191,30 +195,6 @@ def _oblique_pair(Env_big, Env_small, Dcut, shape_big, shape_small, dt):
     Px = ops.gemm(ops.contiguous(u[:, :chi].transpose(0, 1)), RX)
     ops.scale_rows_(Px, sis, chi, n, 2)                                # s^-1/2 * (rows)
     return (ops.cast(Px, dt).view((chi,) + shape_small), ops.cast(PX, dt).view(shape_big + (chi,)))
-
-
-def _truncate_with_tail(s, P, k):
-    """truncate() when only the leading singular values of P are known: the
-    discarded mass ||P||_F^2 - sum s_i^2 enters as one extra trailing value,
-    so the tail-norm rule sees the exact total (the degeneracy rule only
-    looks at positions <= k, inside the known part)."""
-    n = P.shape[0]
-    dev = s.device
-    Pc = ops.contiguous(P)
-    rows = ops.gemm(Pc.view(n, 1, n), Pc.view(n, n, 1)).view(1, n)          # squared row norms
-    ones = ops.fill_(torch.empty((n, 1), dtype=torch.float64, device=dev), 1.0)
-    tail = ops.gemm(rows, ones)                                               # ||P||_F^2, (1,1)
-    sv = ops.contiguous(s).view(1, -1)
-    s2 = ops.gemm(sv, ops.contiguous(sv.transpose(0, 1)))                     # sum s^2, (1,1)
-    one = ops.fill_(torch.empty((1, 1), dtype=torch.float64, device=dev), 1.0)
-    ops.gemm(one, s2, tail, alpha=-1.0, beta=1.0)
-    mag = ops.maxabs(tail)                                                    # |.| (rounding)
-    root = ops.fill_(torch.empty(1, dtype=torch.float64, device=dev), 1.0)
-    ops.scale_rows_(root, mag, 1, 1, 1)                                       # sqrt(tail)
-    ext = torch.empty(s.numel() + 1, dtype=torch.float64, device=dev)
-    ops.copy_into(ext[:-1], s)
-    ops.copy_into(ext[-1:], root)
-    return linalg.truncate(ext, k, degeneracy_eps, truncate_eps)
 
 
 def squeezer(A, UM, sM, VMH, D, Dcut):

@@ -478,6 +478,9 @@ extern "C" int trg_eigh_topk_f64(int batch, int n, int k, int l, int q, const do
         const dim3 grid((unsigned)std::min<long long>(cdiv(sX, 256), 4096), batch);
         hipLaunchKernelGGL(axpy_scaled_kernel, grid, 256, 0, s, sX, shift_rel, tr, Q, sX, Y, sX);
         TRG_TRY(check_launch());
+        // every iterate gets all three passes: after a single shifted pass the
+        // next product with a wide-spectrum A is too ill-conditioned for the
+        // unshifted passes (the Cholesky breaks down)
         TRG_TRY(cholqr3(batch, n, l, Y, Q, G, R, flags, info, s));   // result lands in Q
     }
     // Rayleigh-Ritz: H = Q^T A Q, H = W diag(theta) W^T

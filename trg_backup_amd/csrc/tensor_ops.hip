@@ -249,7 +249,7 @@ __global__ void scale_rows_kernel(T *__restrict__ Y, long long n, long long sY, 
         const T v = xb[(i / inner) % period];
         T f;
         if (mode == 0) f = v;
-        else if (mode == 1) f = sqrt(v);
+        else if (mode == 1) f = sqrt(v < (T)0 ? (T)0 : v);   // eigenvalues a rounding below 0 -> 0; NaN stays
         else if (mode == 2) f = (T)1 / sqrt(v);
         else f = (T)1 / v;
         y[i] *= f;

@@ -9,8 +9,8 @@ import ATRG_Tensor as Tensor``; e.g. measurement/Ising_model_3d.py).  After
     trg_backup_amd.dropin.install()
 
 those names resolve to the GPU modules of this package.  Modules of the
-reference outside the path (``trg.TRG_2d``, ``trg.HOTRG_3d_QR``,
-``trg.gauge_fixing``, ...) are not provided and still fail to import.
+reference outside the path (``trg.TRG_2d``, ``trg.gauge_fixing``, ...) are
+not provided and still fail to import.
 """
 import importlib
 import sys
@@ -20,6 +20,7 @@ ALIASES = {
     "trg.HOTRG_2d": "trg_backup_amd.trg.HOTRG_2d",
     "trg.HOTRG_3d": "trg_backup_amd.trg.HOTRG_3d",
     "trg.ATRG_3d_new": "trg_backup_amd.trg.ATRG_3d_new",
+    "trg.HOTRG_3d_QR": "trg_backup_amd.trg.HOTRG_3d_QR",
     "utility": "trg_backup_amd.utility",
     "utility.truncated_svd": "trg_backup_amd.utility.truncated_svd",
     "utility.randomized_svd": "trg_backup_amd.utility.randomized_svd",

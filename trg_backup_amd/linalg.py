@@ -50,9 +50,9 @@ def orthonormalize(X, perturb=PERTURB, stream_id=7, passes=3):
     """Orthonormal basis (B, m, l) of the columns of X (B, m, l), l <= m.
     A relative Gaussian perturbation keeps rank-deficient blocks of full
     column rank (directions below perturb * max|X| are noise either way).
-    passes=1: one shifted CholeskyQR pass, a well-conditioned (not fully
-    orthonormal) basis of the same span, for the inner steps of a power
-    iteration."""
+    passes=1: one shifted CholeskyQR pass, a basis of the same span that is
+    only as well conditioned as the shift allows (not enough to feed a power
+    iteration on a matrix of wide spectrum; the default is CholeskyQR3)."""
     X = ops.contiguous(_f64(X))
     B, m, l = X.shape
     if perturb > 0:
@@ -100,14 +100,13 @@ def svd_topk(A, l, q, seed=RSVD_SEED, Aop=None, ATop=None, shape=None):
     else:
         B, m, n = shape
     Om = ops.gaussian((n, l), seed=seed, device="cuda", batch=B, stream_stride=0).view(B, n, l)
-    # inner iterates only need a well-conditioned basis of their span (one
-    # shifted CholeskyQR pass); the basis the result is built from is the
-    # fully orthonormalised last one
-    inner = dict(perturb=0.0, passes=1)
-    Q = orthonormalize(Aop(Om), **(inner if q > 0 else {}))
-    for it in range(q):
-        Z = orthonormalize(ATop(Q), **inner)
-        Q = orthonormalize(Aop(Z), **(inner if it < q - 1 else {}))
+    # every iterate is fully orthonormalised (CholeskyQR3): a single shifted
+    # pass leaves a basis whose next product with a matrix of wide spectrum
+    # is too ill-conditioned for the unshifted passes of the final block
+    Q = orthonormalize(Aop(Om))
+    for _ in range(q):
+        Z = orthonormalize(ATop(Q))
+        Q = orthonormalize(Aop(Z))
     Bt = ATop(Q)                                   # (B, n, l) = (Q^T A)^T
     v, s, uh = svd_thin(Bt)                        # Bt = v s uh  ->  Q^T A = uh^T s v^T
     return gemm(Q, uh.transpose(1, 2)), s, v.transpose(1, 2)
@@ -161,6 +160,53 @@ def truncate(s, k, degeneracy_eps, truncate_eps):
     call("trg_truncate_rank", n, ptr(s), int(k), float(degeneracy_eps), float(truncate_eps), ptr(chi), ptr(err),
          stream_ptr())
     return int(chi.item()), float(err.item())
+
+
+def truncate_known(s, tail_sq, total_sq, k, degeneracy_eps, truncate_eps):
+    """truncate() over the leading values s of a longer spectrum whose
+    remaining squares sum to tail_sq (device scalars tail_sq and total_sq =
+    the sum of all squares): the rest enters as one trailing value
+    sqrt(tail_sq), so the tail-norm rule of utility/truncated_svd.py:57-61
+    sees the exact total; the degeneracy rule only looks at positions <= k,
+    which must lie inside s.  A tail within rounding of the total (the
+    difference of two sums that agree to the last bits) counts as zero, so an
+    exactly rank-deficient spectrum keeps the reference's cut."""
+    buf = torch.empty(2, dtype=torch.float64, device=s.device)
+    ops.copy_into(buf[0:1], _f64(tail_sq).reshape(1))
+    ops.copy_into(buf[1:2], _f64(total_sq).reshape(1))
+    t2, tot = buf.tolist()
+    if t2 <= 64 * 2.220446049250313e-16 * abs(tot):
+        return truncate(s, k, degeneracy_eps, truncate_eps)
+    ext = torch.empty(s.numel() + 1, dtype=torch.float64, device=s.device)
+    ops.copy_into(ext[:-1], s)
+    ops.fill_(ext[-1:], math.sqrt(t2))
+    return truncate(ext, k, degeneracy_eps, truncate_eps)
+
+
+def frobenius_sq(P):
+    """||P||_F^2 of a float64 matrix as a (1, 1) device tensor (row norms by a
+    batched dot, then their sum)."""
+    m, n = P.shape
+    Pc = ops.contiguous(P)
+    rows = gemm(Pc.view(m, 1, n), Pc.view(m, n, 1)).view(1, m)
+    ones = ops.fill_(torch.empty((m, 1), dtype=torch.float64, device=P.device), 1.0)
+    return gemm(rows, ones)
+
+
+def sum_sq(s):
+    """sum s_i^2 of a device vector as a (1, 1) device tensor."""
+    sv = ops.contiguous(_f64(s)).view(1, -1)
+    return gemm(sv, ops.contiguous(sv.transpose(0, 1)))
+
+
+def truncate_with_tail(s, P, k, degeneracy_eps, truncate_eps):
+    """truncate() when only the leading singular values s of the float64
+    matrix P are known: the discarded mass ||P||_F^2 - sum s_i^2 is the tail."""
+    tot = frobenius_sq(P)
+    tail = ops.contiguous(tot)
+    one = ops.fill_(torch.empty((1, 1), dtype=torch.float64, device=s.device), 1.0)
+    gemm(one, sum_sq(s), tail, alpha=-1.0, beta=1.0)
+    return truncate_known(s, tail, tot, k, degeneracy_eps, truncate_eps)
 
 
 def count_above(s, eps):

@@ -1,0 +1,187 @@
+"""Drop-in for the reference's trg/HOTRG_3d_QR.py, computed on the GPU by libtrg.
+
+Same names and arguments as trg/HOTRG_3d_QR.py:29-795 for the pure-tensor
+flow.  Configuration the reference reads at import (:19-27) is kept as module
+globals with defaults instead of required inputs: `select_squeezer` from
+HOTRG_SQUEEZER (default "QR"), `degeneracy_eps` from DGENERACY_EPS (default
+0), `truncate_eps` = 1e-10, and `slicing` (sys.argv[4] * 5 in the reference),
+which only splits the reference's contraction loop and does not change the
+result.  The diagnostics written on every step (cal_X, cal_coorelation,
+cal_scaling_dimensions, prints and timings) are not emitted.  The squeezers
+take the pure-tensor form (T0 = T1 = T2 = T3); the impurity flows
+(:428-557, :798-1033) are not on the device.
+"""
+import math
+import os
+
+import torch
+
+from .. import hotrg, hotrg_qr, ops
+from .._io import from_device, to_device
+from .._lib import TrgError
+
+select_squeezer = os.environ.get("HOTRG_SQUEEZER", "QR")
+degeneracy_eps = float(os.environ.get("DGENERACY_EPS", "0") or 0)
+truncate_eps = hotrg_qr.TRUNCATE_EPS
+slicing = None
+PRECISION = hotrg.ALL_FP64
+
+
+def set_precision(policy):
+    """Per-step precision of the device flow (default: float64 throughout)."""
+    global PRECISION
+    PRECISION = policy
+
+
+def leg_transposition(T, do_what="transpose", direction="Z"):
+    """trg/HOTRG_3d_QR.py:29-72."""
+    Td, kind = to_device(T)
+    f = hotrg.rearrange3d if do_what == "transpose" else hotrg.restore3d
+    return from_device(f(Td[None], direction)[0], kind)
+
+
+def _pure(T0, T1, T2, T3):
+    Td, kind = to_device(T0)
+    for other in (T1, T2, T3):
+        if other is not T0:
+            Od, _ = to_device(other)
+            if Od.shape != Td.shape or not torch.equal(Od, Td):
+                raise TrgError("the device squeezers take the pure-tensor form T0 = T1 = T2 = T3 "
+                               "(trg/HOTRG_3d_QR.py:412); impurity flows are not on the device")
+    return ops.contiguous(Td[None]), kind
+
+
+def _squeezer(kind_sq, T0, T1, T2, T3, Dcut):
+    Tb, kind = _pure(T0, T1, T2, T3)
+    groups = hotrg_qr.squeeze(Tb, Dcut, kind_sq, degeneracy_eps, hotrg.FP64 if Tb.dtype == torch.float64
+                              else hotrg.FP32)
+    (_, U1a, U1b, U2a, U2b), = groups
+    LX = hotrg.env3d(Tb, 1)
+    TdagT = ops.trace_pairs(LX[0])
+    Px = ops.contiguous(U1a[0].permute(2, 0, 1))
+    PY = ops.contiguous(U2b[0].permute(2, 0, 1))
+    tdt = float(TdagT[0].item()) if kind == "numpy" else TdagT[0]
+    return (from_device(Px, kind), from_device(U1b[0], kind), from_device(U2a[0], kind), from_device(PY, kind), tdt)
+
+
+def squeezer_fix(T0, T1, T2, T3, Dcut):
+    """trg/HOTRG_3d_QR.py:74-118 -> (Px, PX, Py, PY, TdagT)."""
+    return _squeezer("fix", T0, T1, T2, T3, Dcut)
+
+
+def squeezer_original(T0, T1, T2, T3, Dcut):
+    """trg/HOTRG_3d_QR.py:120-193 -> (Px, PX, Py, PY, TdagT)."""
+    return _squeezer("original", T0, T1, T2, T3, Dcut)
+
+
+def squeezer_QR(T0, T1, T2, T3, Dcut):
+    """trg/HOTRG_3d_QR.py:195-281 -> (Px, PX, Py, PY, TdagT)."""
+    return _squeezer("QR", T0, T1, T2, T3, Dcut)
+
+
+def squeezer(T0, T1, T2, T3, Dcut, squeezer_type):
+    """trg/HOTRG_3d_QR.py:394-406."""
+    return _squeezer(squeezer_type, T0, T1, T2, T3, Dcut)
+
+
+def slice_legs(leg_size, slicing):
+    """trg/HOTRG_3d_QR.py:284-316 (host logic): `slicing` contiguous slices of
+    range(leg_size), one per index when leg_size < slicing."""
+    leg_size, slicing = int(leg_size), int(slicing)
+    if leg_size == 0:
+        raise ValueError("leg_size or slicing is zero!")
+    if leg_size < slicing:
+        return [slice(i, i + 1) for i in range(leg_size)], leg_size
+    if slicing <= 1:
+        return [slice(0, leg_size)], 1
+    short = leg_size // slicing
+    n_short = slicing - leg_size % slicing
+    out, start = [], 0
+    for j in range(slicing):
+        size = short if j < n_short else short + 1
+        out.append(slice(start, start + size))
+        start += size
+    return out, len(out)
+
+
+def coarse_graining(T0, T1, Px, PX, Py, PY, slicing=None, Dcut=None):
+    """adhezi,bcgfiZ,xab,dcX,hgy,Yef->xXyYzZ (trg/HOTRG_3d_QR.py:318-391) with
+    the batched 3D merge; `slicing` does not change the result."""
+    (A, kind), (Bt, _) = to_device(T0), to_device(T1)
+    P = [to_device(p)[0] for p in (Px, PX, Py, PY)]
+    dt = A.dtype
+    U1a = ops.contiguous(ops.cast(P[0], dt).permute(1, 2, 0))[None]
+    U1b = ops.contiguous(ops.cast(P[1], dt))[None]
+    U2a = ops.contiguous(ops.cast(P[2], dt))[None]
+    U2b = ops.contiguous(ops.cast(P[3], dt).permute(1, 2, 0))[None]
+    out = hotrg.merge3d(ops.contiguous(A[None]), ops.contiguous(ops.cast(Bt, dt)[None]), U1a, U1b, U2a, U2b)
+    return from_device(out[0], kind)
+
+
+def normalization(T):
+    """trg/HOTRG_3d_QR.py:715-719 -> (T / c, c), c = |Tr T| over (xx)(yy)(zz)."""
+    Td, kind = to_device(T)
+    Tb = ops.contiguous(Td[None])
+    c = ops.trace_pairs(Tb)
+    ops.normalize_(Tb, c)
+    return from_device(Tb[0], kind), abs(float(c[0].item()))
+
+
+def new_pure_tensor(T, Dcut, direction):
+    """trg/HOTRG_3d_QR.py:408-425 on the dict {'Tensor', ...}."""
+    Td, kind = to_device(T["Tensor"])
+    groups = hotrg_qr.step(ops.contiguous(Td[None]), Dcut, direction, select_squeezer, degeneracy_eps,
+                           hotrg.FP64 if Td.dtype == torch.float64 else hotrg.FP32)
+    T["Tensor"] = from_device(groups[0][1][0], kind)
+    return T
+
+
+def exec_rgstep_pure_tensor(T, Dcut, direction, rgstep):
+    """trg/HOTRG_3d_QR.py:734-745."""
+    key = f"({rgstep['X']},{rgstep['Y']},{rgstep['Z']})"
+    T = new_pure_tensor(T, Dcut, direction)
+    T["Tensor"], c = normalization(T["Tensor"])
+    T["factor"][key] = math.log(c) / 2 ** sum(rgstep.values())
+    return T
+
+
+def pure_tensor(T, Dcut, TOT_RGSTEPS):
+    """trg/HOTRG_3d_QR.py:747-795: the whole flow on the device (one instance,
+    no host round trip per step); fills T['factor'] with the reference's
+    "(X,Y,Z)" keys and returns T."""
+    Td, kind = to_device(T["Tensor"])
+    groups = hotrg_qr.renorm(Td[None], Dcut, TOT_RGSTEPS, select_squeezer, degeneracy_eps, PRECISION)
+    (_, Tn, lnf), = groups
+    for key, v in zip(hotrg_qr.factor_keys(TOT_RGSTEPS), lnf[0].tolist()):
+        T["factor"][key] = v
+    T["Tensor"] = from_device(Tn[0], kind)
+    return T
+
+
+def ln_Z_over_V(T, Dcut, TOT_RGSTEPS):
+    """ln Z / V as measurement/Ising_model_3d.py:101-137 assembles it from
+    pure_tensor (sum of the factors + ln Tr T / 2^N), on the device."""
+    Td, _ = to_device(T)
+    return float(hotrg_qr.ln_z_over_v(Td[None], Dcut, TOT_RGSTEPS, select_squeezer, degeneracy_eps,
+                                      PRECISION)[0].item())
+
+
+def _not_on_device(name, lines):
+    def f(*args, **kwargs):
+        raise NotImplementedError(f"HOTRG_3d_QR.{name} (trg/HOTRG_3d_QR.py:{lines}) is not on the device")
+    f.__name__ = name
+    return f
+
+
+new_impuer_tensor_2points = _not_on_device("new_impuer_tensor_2points", "428-475")
+new_impuer_tensor_absorb = _not_on_device("new_impuer_tensor_absorb", "477-518")
+new_impuer_tensor_1point = _not_on_device("new_impuer_tensor_1point", "520-557")
+cal_X = _not_on_device("cal_X", "559-583")
+find_1stto3rdmax = _not_on_device("find_1stto3rdmax", "585-611")
+cal_scaling_dimensions = _not_on_device("cal_scaling_dimensions", "613-637")
+cal_coorelation = _not_on_device("cal_coorelation", "639-713")
+cal_field_expected_value = _not_on_device("cal_field_expected_value", "798-831")
+exec_rgstep_1point_func = _not_on_device("exec_rgstep_1point_func", "833-857")
+one_point_function = _not_on_device("one_point_function", "859-908")
+exec_rgstep_2point_func = _not_on_device("exec_rgstep_2point_func", "910-995")
+two_point_function = _not_on_device("two_point_function", "997-1033")

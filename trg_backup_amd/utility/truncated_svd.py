@@ -49,9 +49,8 @@ def svd(A, shape, k, truncate_eps=0, degeneracy_eps=0, split=False, return_err=F
             raise TrgError(f"svd: rank {k} of a {m}x{n} matrix is beyond the device SVD limits")
         u, s, vh = linalg.svd_topk(ops.cast(M, torch.float64)[None], l, 12)
         u, s, vh = u[0], s[0], vh[0]
-        from ..atrg import _truncate_with_tail
-        chi, error = _truncate_with_tail(s, ops.cast(M, torch.float64), k) if (degeneracy_eps or truncate_eps) \
-            else (k, 0.0)
+        chi, error = linalg.truncate_with_tail(s, ops.cast(M, torch.float64), k, degeneracy_eps, truncate_eps) \
+            if (degeneracy_eps or truncate_eps) else (k, 0.0)
     chi = min(chi, s.numel())
     u = ops.contiguous(u[:, :chi])
     s = ops.contiguous(s[:chi])
