@@ -58,7 +58,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2, help="warm-up steps per precision tier")
     ap.add_argument("--chi", type=int, default=None, help="hotrg2d 64, hotrg3d 16, atrg3d 64")
     ap.add_argument("--nbeta", type=int, default=None, help="hotrg2d 32, hotrg3d 32 (of the 128-value draw)")
-    ap.add_argument("--fp64-steps", type=int, default=None, help="hotrg2d 6, hotrg3d 4, atrg3d 16")
+    ap.add_argument("--fp64-steps", type=int, default=None,
+                    help="hotrg2d 6, hotrg3d 4, atrg3d 16, hotrg3d_qr all (its oblique projectors amplify "
+                         "float32 rounding: f32 after step 4 gives 5e-8)")
     ap.add_argument("--atrg-q", type=int, default=2)
     ap.add_argument("--squeezer", default="QR", choices=["original", "QR", "fix"], help="hotrg3d_qr squeezer")
     ap.add_argument("--atrg-prec", default="f32", choices=["f32", "f64"])
@@ -204,7 +206,8 @@ def bench_hotrg3d_qr(args):
     """Batched 3D Ising HOTRG with squeezers (trg/HOTRG_3d_QR.py, the 3D HOTRG the reference's 3D
     drivers run): beta from numpy.random.default_rng(0).uniform(0.20, 0.24, 128) (the first --nbeta),
     chi = 16, steps in the "ZXY" cycle from the initial tensor, squeezer --squeezer; float64 for the
-    first --fp64-steps steps, then float32 MFMA merges with float64 environments and projectors."""
+    first --fp64-steps steps (default: every step), then float32 MFMA merges with float64
+    environments and projectors."""
     import numpy as np
     import torch
 
@@ -221,21 +224,31 @@ def bench_hotrg3d_qr(args):
 
     autotune.enable(not args.no_tune)
     t0 = time.perf_counter()
-    warm = hotrg_qr.schedule({"X": K, "Y": K, "Z": K})[:min(K, args.fp64_steps + 2)]
+    warm = hotrg_qr.schedule({"X": K, "Y": K, "Z": K})[:min(K, max(args.fp64_steps, 3) + 1)]
     hotrg_qr.renorm(T0, chi, {d: warm.count(d) for d in "XYZ"}, args.squeezer, 0.0, policy)
     torch.cuda.synchronize()
     autotune.enable(False)
     log(f"warm-up in {time.perf_counter() - t0:.1f} s")
     torch.cuda.empty_cache()
-    rec = ops.GemmRecorder("merge3d_x3")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    with rec:
-        z = hotrg_qr.ln_z_over_v(T0, chi, tot, args.squeezer, 0.0, policy)
-        torch.cuda.synchronize()
+    groups = hotrg_qr.renorm(T0, chi, tot, args.squeezer, 0.0, policy)
+    z = torch.empty(nb, dtype=torch.float64, device=dev)
+    for idx, Tg, lg in groups:
+        zg = hotrg.lnz3d(Tg, lg, K)
+        for j, b in enumerate(idx):
+            ops.copy_into(z[b:b + 1], zg[j:j + 1])
+    torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     zt = z.cpu().numpy()
-    log(f"timed {K} steps x {nb} instances in {dt:.2f} s")
+    log(f"timed {K} steps x {nb} instances in {dt:.2f} s ({len(groups)} rank group(s))")
+    # the chi^11 GEMM's launch time at the last step's precision, from one more steady step
+    rec = ops.GemmRecorder("merge3d_x3")
+    with rec:
+        idx, Tg, _ = groups[0]
+        hotrg_qr.step(Tg, chi, order[-1], args.squeezer, 0.0, policy(K))
+        torch.cuda.synchronize()
+    del groups, Tg
     extra = {"lnZ_per_site": {"betas": betas[:3], "values": zt[:3].tolist()}, "squeezer": args.squeezer}
     if not args.no_validate:
         sel = [0, nb - 1] if nb > 1 else [0]
@@ -244,8 +257,9 @@ def bench_hotrg3d_qr(args):
     cfg = {"workload": f"3D Ising HOTRG_3d_QR ({args.squeezer} squeezer) chi={chi}, {K} RG steps (ZXY cycle) "
                        f"from the initial tensor, {nb} beta from default_rng(0).uniform(0.20,0.24) batched",
            "chi": chi, "nbeta": nb, "rg_steps": K, "policy": policy.describe(), "parallelism": "single GPU, batched"}
-    roof = _roof(rec, "merge chi^11 GEMM (T.Px.Py x T, f32 MFMA)", PEAK_TFLOPS["f32"])
-    print(json.dumps(_line(cfg, nb * K / dt, K, args.warmup, dt, "f32", roof, extra)), flush=True)
+    last = "f64" if policy(K).store == torch.float64 else "f32"
+    roof = _roof(rec, f"merge chi^11 GEMM (T.Px.Py x T, {last} MFMA)", PEAK_TFLOPS[last])
+    print(json.dumps(_line(cfg, nb * K / dt, K, args.warmup, dt, last, roof, extra)), flush=True)
     return 0
 
 
@@ -406,7 +420,7 @@ def main():
     args.nbeta = args.nbeta or (1 if wl == "atrg3d" else 32)
     args.steps = args.steps or {"hotrg2d": 40, "hotrg3d": 6, "hotrg3d_qr": 6, "atrg3d": 30}.get(wl, 40)
     args.fp64_steps = (args.fp64_steps if args.fp64_steps is not None
-                       else {"hotrg2d": 6, "hotrg3d": 4, "hotrg3d_qr": 4, "atrg3d": 16}.get(wl, 0))
+                       else {"hotrg2d": 6, "hotrg3d": 4, "hotrg3d_qr": 10 ** 6, "atrg3d": 16}.get(wl, 0))
     if args.traffic_probe:
         return traffic_probe(args)
     if wl == "hotrg3d":

@@ -288,3 +288,30 @@ def _gemm_cfg(cfg, A, B, C):
     _lib.call("trg_gemm_ex", cfg, ops.dtype_code(A), ops.dtype_code(C), _lib.BF16, bt, M, N, K, 1.0,
               A.data_ptr(), A.stride(0), A.stride(1), A.stride(2), B.data_ptr(), B.stride(0), B.stride(1),
               B.stride(2), 0.0, C.data_ptr(), M * N, N, 1, 0, 0, 0, 0, ops.stream_ptr())
+
+
+@pytest.mark.parametrize("cc,cdt", [(None, torch.float32), ("bf16", torch.float32), ("bf16x3", torch.float32),
+                                    ("bf16", torch.bfloat16)])
+def test_gemm_row_contiguous_output_vector_epilogue(gpu, cc, cdt):
+    """A destination whose rows are contiguous (rsC = 1) takes the 4-row vector
+    epilogue; it equals, bit for bit, the same product written row-major
+    (scalar epilogue), with an M that is not a multiple of 4 (edge groups fall
+    back to scalars) and with beta != 0 (read-modify-write)."""
+    from trg_backup_amd import ops
+    g = torch.Generator(device="cpu").manual_seed(11)
+    B, M, N, K = 3, 70, 45, 37
+    A = torch.randn(B, M, K, generator=g).to(gpu)
+    Bm = torch.randn(B, K, N, generator=g).to(gpu)
+    C0 = torch.randn(B, M, N, generator=g).to(cdt).to(gpu)
+    Cv_buf = torch.empty(B, N, 72, dtype=cdt, device=gpu)      # rows padded to a multiple of 4
+    Cv = Cv_buf[:, :, :M].transpose(1, 2)                        # (B, M, N) view, row stride 1
+    Cv.copy_(C0)
+    Cs = C0.clone()                                              # row-major: scalar epilogue
+    assert Cv.stride(1) == 1 and Cv.stride(2) % 4 == 0
+    ops.gemm(A, Bm, Cv, alpha=1.5, beta=0.5, compute=cc)
+    ops.gemm(A, Bm, Cs, alpha=1.5, beta=0.5, compute=cc)
+    torch.cuda.synchronize()
+    assert torch.equal(Cv.contiguous(), Cs)
+    ref = 1.5 * torch.matmul(A.double(), Bm.double()) + 0.5 * C0.double()
+    tol = 1e-4 if (cc is None and cdt == torch.float32) else 5e-2
+    assert (Cs.double() - ref).abs().max().item() < tol * ref.abs().max().item()

@@ -64,8 +64,9 @@ def config_name(dt, cc, config, cdt=None):
     return _lib.load().trg_gemm_config_name(dt, dt if cdt is None else cdt, cc, config).decode()
 
 
-def _key(dt, cdt, cc, batch, M, N, K, amc, bnc, aligned):
-    return f"{dt}:{cdt}:{cc}:{batch}:{M}:{N}:{K}:{int(amc)}{int(bnc)}:{int(aligned)}"
+def _key(dt, cdt, cc, batch, M, N, K, amc, bnc, aligned, layout=""):
+    k = f"{dt}:{cdt}:{cc}:{batch}:{M}:{N}:{K}:{int(amc)}{int(bnc)}:{int(aligned)}"
+    return k + (f":{layout}" if layout else "")
 
 
 def save(path):
@@ -90,33 +91,40 @@ def _aligned(args):
     return ok(A, sA, rsA if rsA != 1 else csA) and ok(B, sB, rsB if rsB != 1 else csB)
 
 
-def choose(dt, cc, batch, M, N, K, operands, launch, out=None, cdt=None):
+def choose(dt, cc, batch, M, N, K, operands, launch, out=None, cdt=None, layout="", like=None):
     """Config for this problem (-1 = heuristic).  `launch(config, C)` runs
-    the GEMM with beta = 0 into a (batch, M, N) contiguous buffer C; it is
-    called only while tuning, into `out` when the caller's output may be
-    overwritten (beta = 0, contiguous), else into a scratch buffer."""
+    the GEMM with beta = 0 into a buffer C laid out like the caller's
+    destination (same strides and output index, so the epilogue under test is
+    the one the call will use); it is called only while tuning, into `out`
+    when the caller's output may be overwritten (beta = 0), else into a
+    scratch buffer shaped like `like` (default: a (batch, M, N) contiguous
+    buffer).  `layout` names the destination layout in the table key."""
     A, sA, rsA, csA, B, sB, rsB, csB = operands
     cdt = dt if cdt is None else cdt
-    key = _key(dt, cdt, cc, batch, M, N, K, rsA == 1, csB == 1, _aligned(operands))
+    key = _key(dt, cdt, cc, batch, M, N, K, rsA == 1, csB == 1, _aligned(operands), layout)
     with _LOCK:
         hit = _TABLE.get(key)
     if hit is not None or not _STATE["enabled"]:
         return -1 if hit is None else hit
     if torch.cuda.is_current_stream_capturing():
         return -1
-    best = _tune(dt, cdt, cc, batch, M, N, K, launch, out)
+    best = _tune(dt, cdt, cc, batch, M, N, K, launch, out, like)
     with _LOCK:
         _TABLE[key] = best
     return best
 
 
-def _tune(dt, cdt, cc, batch, M, N, K, launch, out):
+def _tune(dt, cdt, cc, batch, M, N, K, launch, out, like=None):
     n = config_count(dt, cc, cdt)
     if n <= 1:
         return -1
     dtype = {_lib.F64: torch.float64, _lib.BF16: torch.bfloat16}.get(cdt, torch.float32)
-    scratch = out if out is not None else torch.empty((batch, M, N), dtype=dtype,
-                                                       device=torch.cuda.current_device())
+    if out is not None:
+        scratch = out
+    elif like is not None:
+        scratch = torch.empty_strided(tuple(like.size()), tuple(like.stride()), dtype=dtype, device=like.device)
+    else:
+        scratch = torch.empty((batch, M, N), dtype=dtype, device=torch.cuda.current_device())
     best, best_ms = -1, float("inf")
     for cfg in range(n):
         launch(cfg, scratch)                       # warm-up (and code object load)

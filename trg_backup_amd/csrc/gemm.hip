@@ -21,6 +21,8 @@ namespace trg {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct GemmArgs {
     int M, N, K;
@@ -38,6 +40,10 @@ struct GemmArgs {
     // so a GEMM can write straight into a permuted tensor layout.
     int cdm, cdn;
     long long csm, csn;
+    // set by the host when rows of C are contiguous (rsC == 1) and every
+    // group of four consecutive rows starts on a 4-element boundary: the
+    // epilogue then writes 4-row vectors
+    int cvec;
 };
 
 __device__ inline long long c_row_offset(const GemmArgs &g, int m) {
@@ -60,6 +66,54 @@ __device__ inline void store_c(const GemmArgs &g, TC *__restrict__ C, Acc (&acc)
     for (int j = 0; j < TN; ++j) {
         const int n = col(j);
         on[j] = n < g.N ? c_col_offset(g, n) : -1;
+    }
+    if constexpr (NACC == 16) {
+        // 32x32 MFMA accumulators: registers 4q..4q+3 of a lane hold four
+        // consecutive rows of one column.  With contiguous rows they leave as
+        // one 4-element vector store (16 B float, 8 B bf16) instead of four
+        // scalar ones; a group cut by the edge of M falls back to scalars.
+        typedef TC vec4 __attribute__((ext_vector_type(4)));
+        if (g.cvec) {
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int m = row(i, 4 * q);
+                    if (m >= g.M) continue;
+                    const long long om = c_row_offset(g, m);
+                    if (m + 3 < g.M) {
+#pragma unroll
+                        for (int j = 0; j < TN; ++j) {
+                            if (on[j] < 0) continue;
+                            vec4 *p = reinterpret_cast<vec4 *>(C + om + on[j]);
+                            vec4 old;
+                            if (g.beta != 0.0) old = *p;
+                            vec4 out;
+#pragma unroll
+                            for (int t = 0; t < 4; ++t) {
+                                TA v = alpha * acc[i][j][4 * q + t];
+                                if (g.beta != 0.0) v += beta * (TA)old[t];
+                                out[t] = (TC)v;
+                            }
+                            *p = out;
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            if (m + t >= g.M) continue;
+#pragma unroll
+                            for (int j = 0; j < TN; ++j) {
+                                if (on[j] < 0) continue;
+                                const long long idx = om + t + on[j];
+                                TA v = alpha * acc[i][j][4 * q + t];
+                                if (g.beta != 0.0) v += beta * (TA)C[idx];
+                                C[idx] = (TC)v;
+                            }
+                        }
+                    }
+                }
+            return;
+        }
     }
 #pragma unroll
     for (int i = 0; i < TM; ++i)
@@ -444,8 +498,6 @@ gemm_bf16_kernel(GemmArgs g) {
 // (a vector is then either fully inside or fully outside K).  Same MFMA
 // instruction and k order as gemm_bf16_kernel, so the two agree bitwise.
 // ---------------------------------------------------------------------------
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // DB: two LDS stages; the next slice is written into the stage the waves are
 // not reading, so one barrier per K slice suffices instead of two.
@@ -678,6 +730,17 @@ static int launch_v(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
 
 // 16-byte vector loads are legal when every vector starts on a 16-byte
 // boundary and never straddles the operand's edge.
+// 4-row vector stores in the epilogue: rows of C contiguous and every place a
+// group of four consecutive rows (m a multiple of 4) can start a multiple of
+// 4 elements, on a base aligned to the vector.
+static int cvec_ok(const GemmArgs &g, int c_dtype) {
+    const int esize = c_dtype == TRG_F64 ? 8 : (c_dtype == TRG_BF16 ? 2 : 4);
+    if (g.rsC != 1) return 0;
+    if (g.cdm && (g.cdm % 4 || g.csm % 4)) return 0;
+    if (g.csC % 4 || (g.cdn && g.csn % 4) || g.sCb % 4) return 0;
+    return reinterpret_cast<uintptr_t>(g.C) % (4 * esize) == 0 ? 1 : 0;
+}
+
 static bool vec_ok(const GemmArgs &g, bool amc, bool bnc, int esize, int vec) {
     auto al = [&](const void *p, long long sb) {
         return (reinterpret_cast<uintptr_t>(p) % (esize * vec) == 0) && (sb % vec == 0);
@@ -773,6 +836,9 @@ static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmAr
             case 5: return launch_bf16in<256, 256, 32, 128, 64, true>(g, batch, amc, bnc, s);
             case 6: return launch_bf16in<256, 128, 64, 64, 64, true>(g, batch, amc, bnc, s);
             case 7: return launch_bf16in<128, 256, 32, 64, 64>(g, batch, amc, bnc, s);
+            case 8: return launch_bf16in<256, 256, 64, 128, 64, true>(g, batch, amc, bnc, s);
+            case 9: return launch_bf16in<256, 256, 128, 128, 64>(g, batch, amc, bnc, s);
+            case 10: return launch_bf16in<128, 256, 64, 64, 64, true>(g, batch, amc, bnc, s);
             default: return TRG_BAD_ARG;
         }
     }
@@ -874,12 +940,13 @@ extern "C" int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, 
         // C = beta * C: a GEMM with an empty sum.  Use the kernel with K = 0 (loop skipped).
     }
     GemmArgs g{M, N, K, A, sAb, rsA, csA, B, sBb, rsB, csB, C, sCb, rsC, csC, alpha, beta, 0, 0};
+    g.cvec = cvec_ok(g, dtype);
     if (dtype == TRG_BF16) return TRG_UNSUPPORTED;   // bf16 storage: trg_gemm_ex
     return gemm_dispatch(dtype, dtype, compute, -1, g, batch, amc, bnc, s);
 }
 
 extern "C" int trg_gemm_config_count(int dtype, int c_dtype, int compute) {
-    if (dtype == TRG_BF16 && c_dtype == TRG_F32 && compute == TRG_BF16) return 8;
+    if (dtype == TRG_BF16 && c_dtype == TRG_F32 && compute == TRG_BF16) return 11;
     if (dtype == TRG_F32 && c_dtype == TRG_BF16 && compute == TRG_BF16) return 3;
     if (c_dtype != dtype) return 0;
     if (dtype == TRG_F32 && compute == TRG_F32) return 8;
@@ -901,7 +968,8 @@ extern "C" const char *trg_gemm_config_name(int dtype, int c_dtype, int compute,
                                 "128x256x32 w64x64 db"};
     static const char *bin[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "256x256x32 w128x64",
                                 "256x256x64 w128x64", "256x256x32 w128x64 db", "256x128x64 w64x64 db",
-                                "128x256x32 w64x64"};
+                                "128x256x32 w64x64", "256x256x64 w128x64 db", "256x256x128 w128x64",
+                                "128x256x64 w64x64 db"};
     static const char *f64[] = {"128x128x16 w64x64", "64x64x16 w32x32", "128x64x16 w64x32", "64x128x16 w32x64"};
     if (config < 0 || config >= trg_gemm_config_count(dtype, c_dtype, compute)) return "";
     if (dtype == TRG_BF16) return bin[config];
@@ -929,5 +997,6 @@ extern "C" int trg_gemm_ex(int config, int dtype, int c_dtype, int compute, int 
     if (!bnc && rsB != 1) return TRG_BAD_ARG;
     GemmArgs g{M, N, K, A, sAb, rsA, csA, B, sBb, rsB, csB, C, sCb, rsC, csC, alpha, beta, 0, 0,
                cdm, cdn, csm, csn};
+    g.cvec = cvec_ok(g, c_dtype);
     return gemm_dispatch(dtype, c_dtype, compute, config, g, batch, amc, bnc, as_stream(stream));
 }

@@ -217,22 +217,27 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
         nc = c1 - c0
         A = T1m[:, :, c0 * Dy * Dy:c1 * Dy * Dy].transpose(1, 2)          # (B, (c,k,e), a)
         Z1p = torch.empty(B * nc * Dy * Dy * Dx * chi_a, dtype=zdt, device=T1.device)
+        # Z1 is computed transposed (Ua^T T1^T): its destination is contiguous
+        # along the Ua index, which then runs over the product's rows, so the
+        # epilogue stores four consecutive rows as one vector
+        UamT, AT = Uam.transpose(1, 2), A.transpose(1, 2)
         if kinner:
-            # [c,k,i,e,b]: rows (c,k | e), columns (i | b)
-            gemm(A, Uam, Z1p, compute=compute,
-                 out_layout=([(nc * Dy, chi_a * Dy * Dx), (Dy, Dx)], [(chi_a, Dy * Dx), (Dx, 1)]))
+            # [c,k,i,e,b]: rows (i | b), columns (c,k | e)
+            gemm(UamT, AT, Z1p, compute=compute,
+                 out_layout=([(chi_a, Dy * Dx), (Dx, 1)], [(nc * Dy, chi_a * Dy * Dx), (Dy, Dx)]))
             Z2a = Z1p.view(B, nc * Dy * chi_a, Dy * Dx)
         else:
-            # [e,b,c,k,i]: rows (c,k | e), columns (b | i)
-            gemm(A, Uam, Z1p, compute=compute,
-                 out_layout=([(nc * Dy, chi_a), (Dy, Dx * nc * Dy * chi_a)], [(Dx, nc * Dy * chi_a), (chi_a, 1)]))
+            # [e,b,c,k,i]: rows (b | i), columns (c,k | e)
+            gemm(UamT, AT, Z1p, compute=compute,
+                 out_layout=([(Dx, nc * Dy * chi_a), (chi_a, 1)], [(nc * Dy, chi_a), (Dy, Dx * nc * Dy * chi_a)]))
             Z2a = Z1p.view(B, Dy * Dx, nc * Dy * chi_a).transpose(1, 2)
         # Z2 [(c,k,i),(l,d)] -> [k,i,l,c,d]: rows (c | k,i), columns (l | d)
         Z2p = torch.empty(B * Dy * chi_a * Dy * nc * Dx, dtype=T1.dtype, device=T1.device)
         gemm(Z2a, Tp, Z2p, compute=compute, tag="merge2d_z2",
              out_layout=([(nc, Dx), (Dy * chi_a, Dy * nc * Dx)], [(Dy, nc * Dx), (Dx, 1)]))
-        gemm(Z2p.view(B, Dy * chi_a * Dy, nc * Dx), Ubm[:, c0 * Dx:c1 * Dx, :], outm,
-             beta=0.0 if c0 == 0 else 1.0, compute=compute)
+        # out^T[j,(k,i,l)] += Ub^T Z2^T: rows j are contiguous in out (vector epilogue)
+        gemm(Ubm[:, c0 * Dx:c1 * Dx, :].transpose(1, 2), Z2p.view(B, Dy * chi_a * Dy, nc * Dx).transpose(1, 2),
+             outm.transpose(1, 2), beta=0.0 if c0 == 0 else 1.0, compute=compute)
         del Z1p, Z2p
     return permute(out, (0, 2, 4, 1, 3))                                   # [i,j,k,l]
 

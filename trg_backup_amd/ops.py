@@ -209,11 +209,15 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_layout=N
         Cstr = (C.stride(0) if batch > 1 else 0, C.stride(1), C.stride(2))
     config = -1
     if autotune.active():
+        # tuned with the call's own destination layout (strides, two-level
+        # index), so the epilogue it times is the one this call runs
         def launch(cfg, out):
             call("trg_gemm_ex", cfg, dt, cdt, cc, batch, M, N, K, float(alpha), *opnds,
-                 0.0, ptr(out), M * N if batch > 1 else 0, N, 1, 0, 0, 0, 0, stream_ptr())
-        own = out_layout is None and beta == 0.0 and C.is_contiguous() and C.shape == (batch, M, N)
-        config = autotune.choose(dt, cc, batch, M, N, K, opnds, launch, C if own else None, cdt=cdt)
+                 0.0, ptr(out), *Cstr, *split, stream_ptr())
+        layout = "" if (Cstr[1], Cstr[2], split) == (N, 1, (0, 0, 0, 0)) else \
+            f"{Cstr[1]},{Cstr[2]},{','.join(str(v) for v in split)}"
+        config = autotune.choose(dt, cc, batch, M, N, K, opnds, launch, C if beta == 0.0 else None, cdt=cdt,
+                                 layout=layout, like=C)
     recs = [r for r in _RECORDERS if tag is not None and r.tag == tag]
     if recs:
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
