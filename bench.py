@@ -41,8 +41,8 @@ sys.path.insert(0, ROOT)
 
 # MI355X published dense MFMA peaks by the MFMA type a kernel issues (split-bf16 issues bf16
 # MFMAs, three per algorithmic product, so its algorithmic flop/s can reach a third of this)
-PEAK_TFLOPS = {"f32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0, "f64": 78.6}
-MFMA_PER_PRODUCT = {"f32": 1, "bf16": 1, "bf16x3": 3, "f64": 1}
+PEAK_TFLOPS = {"f32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0, "bf16x6": 2500.0, "f64": 78.6}
+MFMA_PER_PRODUCT = {"f32": 1, "bf16": 1, "bf16x3": 3, "bf16x6": 6, "f64": 1}
 
 
 def parse():
@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--atrg-q", type=int, default=2)
     ap.add_argument("--squeezer", default="QR", choices=["original", "QR", "fix"], help="hotrg3d_qr squeezer")
     ap.add_argument("--atrg-prec", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--split6-merge-from", type=int, default=6,
+                    help="bf16x6 merge (three-piece split-bf16, fp32 accuracy) after this step (-1: never)")
     ap.add_argument("--split-merge-from", type=int, default=10, help="bf16x3 merge after this step (-1: never)")
     ap.add_argument("--bf16-env-from", type=int, default=12, help="bf16 environments after this step (-1: never)")
     ap.add_argument("--bf16-merge-from", type=int, default=22, help="bf16 merge after this step (-1: never)")
@@ -387,7 +389,7 @@ def traffic_probe(args):
     if args.load_tune:
         autotune.load(args.load_tune)
     autotune.enable(not args.no_tune)
-    for prec in (hotrg.FP32, hotrg.Precision(torch.float32, "f32", "bf16x3"),
+    for prec in (hotrg.FP32, hotrg.Precision(torch.float32, "f32", "bf16x6"), hotrg.Precision(torch.float32, "f32", "bf16x3"),
                  hotrg.Precision(torch.float32, "f32", "bf16")):
         # tune first, then the step whose dispatches are read
         hotrg.step2d(T, chi, d, prec)
@@ -442,7 +444,8 @@ def main():
     betas = [0.40 + 0.08 * i / (nb - 1) for i in range(nb)] if nb > 1 else [0.44]
     opt = lambda v: None if v < 0 else v
     policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32, bf16_env_from=opt(args.bf16_env_from),
-                          split_merge_from=opt(args.split_merge_from), bf16_merge_from=opt(args.bf16_merge_from))
+                          split_merge_from=opt(args.split_merge_from), bf16_merge_from=opt(args.bf16_merge_from),
+                          split6_merge_from=opt(args.split6_merge_from))
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     T0 = torch.stack([models.initial_tensor("ising2d", b, dev) for b in betas])
 
@@ -495,7 +498,7 @@ def main():
     torch.cuda.empty_cache()
 
     # merge GEMM launches grouped by (storage, compute)
-    names = {_lib.F32: "f32", _lib.BF16: "bf16", _lib.BF16X3: "bf16x3", _lib.F64: "f64"}
+    names = {_lib.F32: "f32", _lib.BF16: "bf16", _lib.BF16X3: "bf16x3", _lib.BF16X6: "bf16x6", _lib.F64: "f64"}
     groups = {}
     for (e0, e1, fl), cfg in zip(rec.events, rec.event_configs):
         groups.setdefault(names[cfg[2]], []).append((e0.elapsed_time(e1), fl, cfg))
@@ -535,7 +538,7 @@ def main():
                      "mfma_issue_frac": d["mfma_issue_frac"],
                      "note": "achieved = algorithmic 2MNK flops / live launch time; peak = published dense peak "
                              "of the MFMA type issued; mfma_issue_frac counts the MFMA work issued (bf16x3: "
-                             "3 bf16 products per algorithmic product)",
+                             "3 bf16 products per algorithmic product, bf16x6: 6)",
                      "merge_gemm_by_precision": kern},
     }
     out["lnZ_per_site"] = {"betas": [betas[0], betas[nb // 2], betas[-1]],

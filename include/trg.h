@@ -29,8 +29,9 @@ typedef enum {
 /* Storage types (F32, F64, BF16 for GEMM operands / products and copies;
  * I32 for trg_fill only) and GEMM compute types (F32, F64, BF16, BF16X3 =
  * fp32 operands split into two bf16 parts, three bf16 MFMAs per product,
- * about 16 significant bits per operand). */
-typedef enum { TRG_F32 = 0, TRG_F64 = 1, TRG_BF16 = 2, TRG_I32 = 3, TRG_BF16X3 = 4 } trg_dtype;
+ * about 16 significant bits per operand; BF16X6 = three bf16 parts, six
+ * MFMAs per product, the 24 bits of fp32). */
+typedef enum { TRG_F32 = 0, TRG_F64 = 1, TRG_BF16 = 2, TRG_I32 = 3, TRG_BF16X3 = 4, TRG_BF16X6 = 5 } trg_dtype;
 
 const char *trg_version(void);
 const char *trg_status_string(int status);
@@ -43,9 +44,9 @@ int trg_device_info(int device, int *num_cus, int *lds_per_block, int *warp_size
  * Element (m,k) of A[b] is at A + b*sAb + m*rsA + k*csA (elements, not
  * bytes); likewise B (k,n) and C (m,n).  One of rsA/csA and one of rsB/csB
  * must be 1.  `dtype` is the storage type of A, B and C (F32 or F64);
- * `compute` is F32, BF16 or BF16X3 for F32 storage (BF16: operands rounded
- * to bf16, fp32 accumulation; BF16X3: split-bf16 products) and F64 for F64
- * storage.
+ * `compute` is F32, BF16, BF16X3 or BF16X6 for F32 storage (BF16: operands
+ * rounded to bf16, fp32 accumulation; BF16X3 / BF16X6: split-bf16 products)
+ * and F64 for F64 storage.
  * Replaces every pairwise step of the cuquantum/opt_einsum contractions on
  * the path: trg/HOTRG_2d.py:26-27,65-67; trg/HOTRG_3d.py:42-45,107-116;
  * trg/ATRG_3d_new.py:99-122,232-233,261-265,379-405,424-433.
@@ -61,7 +62,7 @@ int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, double alph
  * the product (c_dtype), an explicit tile configuration (0 <= config <
  * trg_gemm_config_count(dtype, c_dtype, compute); -1 = the shape heuristic)
  * and an optional two-level output index.  Storage combinations:
- * (F32, F32, F32|BF16|BF16X3), (F64, F64, F64), (BF16, F32, BF16) -- bf16
+ * (F32, F32, F32|BF16|BF16X3|BF16X6), (F64, F64, F64), (BF16, F32, BF16) -- bf16
  * operands, both K-contiguous with K and strides multiples of 8 -- and
  * (F32, BF16, BF16), a product rounded to bf16 on the way out.  With cdm > 0 row m of
  * C[b] sits at (m / cdm) * csm + (m % cdm) * rsC, with cdn > 0 column n at

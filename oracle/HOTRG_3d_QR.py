@@ -230,3 +230,80 @@ def ln_Z_over_V(T0, Dcut, TOT_RGSTEPS, squeezer_type="QR", degeneracy_eps=0.0):
     T = pure_tensor({"Tensor": T0, "loc": {}, "factor": {}}, Dcut, dict(TOT_RGSTEPS), squeezer_type, degeneracy_eps)
     V = 2 ** sum(TOT_RGSTEPS.values())
     return float(sum(T["factor"].values()) + np.log(np.einsum("xxyyzz", T["Tensor"])) / V), T
+
+
+# ---------------------------------------------------------------------------
+# one-point function (trg/HOTRG_3d_QR.py:520-557, 798-908)
+# ---------------------------------------------------------------------------
+def new_impuer_tensor_1point(T, T0, loc, Dcut, direction, squeezer_type="QR", degeneracy_eps=0.0):
+    """trg/HOTRG_3d_QR.py:520-557 on arrays: the projectors come from the pure
+    tensor; the impurity T0 is the upper tensor of its pair when its
+    coordinate along `direction` is odd, the lower one when even."""
+    T = leg_transposition(T, "transpose", direction)
+    T0 = leg_transposition(T0, "transpose", direction)
+    Px, PX, Py, PY, _ = squeezer(T, T, T, T, Dcut, squeezer_type, degeneracy_eps)
+    if loc[direction] % 2 == 1:
+        T0 = coarse_graining(T, T0, Px, PX, Py, PY)
+    else:
+        T0 = coarse_graining(T0, T, Px, PX, Py, PY)
+    T = coarse_graining(T, T, Px, PX, Py, PY)
+    return leg_transposition(T, "restore", direction), leg_transposition(T0, "restore", direction)
+
+
+def one_point_function(T, T0, Dcut, TOT_RGSTEPS, squeezer_type="QR", degeneracy_eps=0.0):
+    """trg/HOTRG_3d_QR.py:859-908 on the dicts {'Tensor', 'loc', 'factor'}:
+    T0['factor'] holds ln c0 - ln c per step; returns (T, T0, P) with
+    P = exp(sum T0 factors) Tr T0 / Tr T after the last step
+    (cal_field_expected_value, :798-831)."""
+    rgstep = {"X": 0, "Y": 0, "Z": 0}
+    key = lambda: f"({rgstep['X']},{rgstep['Y']},{rgstep['Z']})"
+    T["Tensor"], c = normalization(T["Tensor"])
+    T0["Tensor"], c0 = normalization(T0["Tensor"])
+    T["factor"][key()] = math.log(c)
+    T0["factor"][key()] = math.log(c0) - math.log(c)
+    for direction in cycle("ZXY"):
+        if sum(rgstep.values()) >= sum(TOT_RGSTEPS.values()):
+            break
+        if rgstep[direction] < TOT_RGSTEPS[direction]:
+            rgstep[direction] += 1
+            T["Tensor"], T0["Tensor"] = new_impuer_tensor_1point(T["Tensor"], T0["Tensor"], T0["loc"], Dcut,
+                                                                 direction, squeezer_type, degeneracy_eps)
+            T["Tensor"], c = normalization(T["Tensor"])
+            T0["Tensor"], c0 = normalization(T0["Tensor"])
+            T["factor"][key()] = math.log(c) / 2 ** sum(rgstep.values())
+            T0["factor"][key()] = math.log(c0) - math.log(c)
+            T0["loc"][direction] //= 2
+    P = math.exp(sum(T0["factor"].values())) * float(np.einsum("xxyyzz", T0["Tensor"])) \
+        / float(np.einsum("xxyyzz", T["Tensor"]))
+    return T, T0, P
+
+
+def network_value(tensors, shape):
+    """Contraction of the periodic box `shape` (Lx, Ly, Lz) of six-leg tensors
+    (x, X, y, Y, z, Z); `tensors` maps site (i, j, k) to its tensor.  Each
+    site's X leg bonds to the x leg of its +x neighbour (likewise y, z); a
+    length-2 axis therefore carries two bonds between the pair.  For tests
+    on boxes of a few sites."""
+    import itertools
+    sites = list(itertools.product(*[range(L) for L in shape]))
+    labels = {}
+
+    def bond(site, d):
+        key = (site, d)
+        if key not in labels:
+            labels[key] = len(labels)
+        return labels[key]
+
+    operands, subs = [], []
+    for s in sites:
+        legs = []
+        for d in range(3):
+            prev = list(s)
+            prev[d] = (prev[d] - 1) % shape[d]
+            legs += [bond(tuple(prev), d), bond(s, d)]
+        operands.append(tensors[s])
+        subs.append(legs)
+    args = []
+    for t, sb in zip(operands, subs):
+        args += [t, sb]
+    return float(np.einsum(*args, [], optimize="greedy"))

@@ -73,3 +73,16 @@ def brute_lnZ_potts3d(beta, q=3, shape=(2, 2, 2)):
     states = np.stack([(n // q ** k) % q for k in range(V)], axis=1)
     E = np.sum(states[:, b[:, 0]] == states[:, b[:, 1]], axis=1)
     return _lnsumexp(beta * E) / V
+
+
+def brute_corr_ising(beta, shape, a, b):
+    """<s_a s_b> of the periodic Ising box ``shape`` (sites a, b as coordinate
+    tuples), by enumeration; same bond convention as brute_lnZ_ising3d."""
+    V, bonds = _torus_bonds(shape)
+    sites = list(itertools.product(*[range(L) for L in shape]))
+    ia, ib = sites.index(tuple(a)), sites.index(tuple(b))
+    states = ((np.arange(2 ** V)[:, None] >> np.arange(V)[None, :]) & 1) * 2 - 1
+    E = np.sum(states[:, bonds[:, 0]] * states[:, bonds[:, 1]], axis=1)
+    w = beta * E
+    p = np.exp(w - np.max(w))
+    return float(np.sum(p * states[:, ia] * states[:, ib]) / np.sum(p))

@@ -96,3 +96,36 @@ def test_hotrg3d_qr_random_tensor_matches_oracle(gpu, kind):
     v = float(hotrg_qr.ln_z_over_v(_dev(T6, gpu)[None], 5, steps, kind, 0.0)[0].item())
     print(f"\n[hotrg3d_qr random {kind}] gpu={v!r} oracle={ref!r} diff={v - ref:.3e}", flush=True)
     assert abs(v - ref) < 1e-11 * abs(ref)
+
+
+@pytest.mark.parametrize("chi,steps", [(16, {"X": 1, "Y": 1, "Z": 1}), (4, {"X": 2, "Y": 1, "Z": 2})])
+def test_hotrg3d_qr_one_point_matches_oracle(gpu, chi, steps):
+    """one_point_function (trg/HOTRG_3d_QR.py:859-908) with a random impurity at
+    (1, 0, 1): the one-point value and ln Z/V against the oracle; untruncated
+    (chi = 16, 2x2x2) also against the direct network contraction."""
+    import itertools
+    from trg_backup_amd.trg import HOTRG_3d_QR as dq
+    old = (dq.select_squeezer, dq.degeneracy_eps)
+    dq.select_squeezer, dq.degeneracy_eps = "QR", 1e-8
+    try:
+        T = models.ising3d_tensor(0.3)
+        T0 = T + 0.3 * np.random.default_rng(4).standard_normal(T.shape)
+        loc = {"X": 1, "Y": 0, "Z": 1}
+        Td, T0d = dq.one_point_function({"Tensor": T, "loc": {}, "factor": {}},
+                                        {"Tensor": T0, "loc": dict(loc), "factor": {}}, chi, dict(steps))
+        n = sum(steps.values())
+        lnz, P = dq.cal_field_expected_value(Td, [T0d], steps, False)
+        _, _, Pref = oq.one_point_function({"Tensor": T, "loc": {}, "factor": {}},
+                                           {"Tensor": T0, "loc": dict(loc), "factor": {}}, chi, dict(steps), "QR",
+                                           1e-8)
+        print(f"\n[one point chi={chi}] gpu={P[0]!r} oracle={Pref!r}", flush=True)
+        assert abs(P[0] - Pref) < 1e-10 * abs(Pref)
+        assert T0d["loc"] == {d: loc[d] >> steps[d] for d in "XYZ"}
+        if chi == 16:
+            sites = list(itertools.product(range(2), range(2), range(2)))
+            Z = oq.network_value({s: T for s in sites}, (2, 2, 2))
+            Zi = oq.network_value({s: (T0 if s == (1, 0, 1) else T) for s in sites}, (2, 2, 2))
+            assert abs(P[0] - Zi / Z) < 1e-10 * abs(Zi / Z)
+            assert abs(lnz - np.log(Z) / 8) < 1e-12
+    finally:
+        dq.select_squeezer, dq.degeneracy_eps = old

@@ -160,7 +160,7 @@ def test_permute_fast_paths(gpu, perm):
 
 
 @pytest.mark.parametrize("dtype,compute", [(torch.float32, None), (torch.float32, "bf16"), (torch.float32, "bf16x3"),
-                                           (torch.float64, None)])
+                                           (torch.float32, "bf16x6"), (torch.float64, None)])
 @pytest.mark.parametrize("shape,ta,tb", [((300, 260, 100), False, False), ((257, 129, 67), True, False),
                                          ((96, 200, 48), False, True), ((64, 4096, 64), True, True),
                                          ((300, 260, 100), False, True), ((130, 70, 67), False, True),
@@ -182,7 +182,7 @@ def test_gemm_tile_configs_bitwise_equal(gpu, dtype, compute, shape, ta, tb):
     tol = {"bf16": 2e-3, "bf16x3": 2e-5}.get(compute, 1e-6 if dtype == torch.float32 else 1e-14)
     assert err < tol, err
     dt = ops.dtype_code(Ad)
-    cc = {"bf16": _lib.BF16, "bf16x3": _lib.BF16X3}.get(compute, dt)
+    cc = {"bf16": _lib.BF16, "bf16x3": _lib.BF16X3, "bf16x6": _lib.BF16X6}.get(compute, dt)
     n = autotune.config_count(dt, cc)
     assert n >= 2
     for cfg in range(n):
@@ -234,7 +234,7 @@ def test_bf16x3_accuracy_between_bf16_and_fp32(gpu):
     assert e["bf16x3"] < 5e-5 and e["bf16x3"] < e["bf16"] / 50 and e[None] < e["bf16x3"]
 
 
-@pytest.mark.parametrize("compute", [None, "bf16", "bf16x3"])
+@pytest.mark.parametrize("compute", [None, "bf16", "bf16x3", "bf16x6"])
 def test_gemm_two_level_output_index(gpu, compute):
     """A product written through a two-level output index equals the plain product
     permuted into the same layout, bit for bit (same kernel, only the epilogue
@@ -291,7 +291,7 @@ def _gemm_cfg(cfg, A, B, C):
 
 
 @pytest.mark.parametrize("cc,cdt", [(None, torch.float32), ("bf16", torch.float32), ("bf16x3", torch.float32),
-                                    ("bf16", torch.bfloat16)])
+                                    ("bf16x6", torch.float32), ("bf16", torch.bfloat16)])
 def test_gemm_row_contiguous_output_vector_epilogue(gpu, cc, cdt):
     """A destination whose rows are contiguous (rsC = 1) takes the 4-row vector
     epilogue; it equals, bit for bit, the same product written row-major
@@ -315,3 +315,22 @@ def test_gemm_row_contiguous_output_vector_epilogue(gpu, cc, cdt):
     ref = 1.5 * torch.matmul(A.double(), Bm.double()) + 0.5 * C0.double()
     tol = 1e-4 if (cc is None and cdt == torch.float32) else 5e-2
     assert (Cs.double() - ref).abs().max().item() < tol * ref.abs().max().item()
+
+
+def test_bf16x6_accuracy_matches_fp32(gpu):
+    """Three-piece split-bf16 products carry the 24 bits of fp32: the error
+    against the float64 product is within a small factor of the fp32 GEMM's
+    and far below the two-piece split.  K is short (32), so the operands'
+    representation, not the fp32 accumulation, sets the error."""
+    from trg_backup_amd import ops
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((2, 256, 32))
+    B = rng.standard_normal((2, 32, 192))
+    Ad, Bd = _dev(A, gpu).float(), _dev(B, gpu).float()
+    Bt = ops.contiguous(Bd.transpose(1, 2)).transpose(1, 2)        # K-contiguous B
+    ref = A.astype(np.float32).astype(np.float64) @ B.astype(np.float32).astype(np.float64)
+    scale = np.abs(ref).max()
+    e = {c: np.abs(ops.gemm(Ad, Bt, compute=c).double().cpu().numpy() - ref).max() / scale
+         for c in (None, "bf16x6", "bf16x3")}
+    print(f"\n[bf16x6] rel err fp32 {e[None]:.2e} bf16x6 {e['bf16x6']:.2e} bf16x3 {e['bf16x3']:.2e}", flush=True)
+    assert e["bf16x6"] < 4 * e[None] and e["bf16x6"] < e["bf16x3"] / 10

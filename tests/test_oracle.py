@@ -264,3 +264,71 @@ def test_hotrg3d_qr_slice_legs():
         assert n == len(parts) == min(leg, max(sl, 1))
         covered = [i for p in parts for i in range(p.start, p.stop)]
         assert covered == list(range(leg))
+
+
+# ---------------------------------------------------------------------------
+# impurity flows (two-point functions) against enumeration
+# ---------------------------------------------------------------------------
+def _spin_impurity(beta, nlegs, shift=0.0):
+    """sum_s (shift + s) prod_leg W_{s,leg}: the site tensor with the spin
+    (plus a constant, so that a trace normalisation sees a non-zero trace) inserted."""
+    W = models.ising_W(beta)
+    sig = np.array([1.0, -1.0]) + shift
+    T = np.zeros((2,) * nlegs)
+    for s in range(2):
+        t = W[s]
+        for _ in range(nlegs - 1):
+            t = np.multiply.outer(t, W[s])
+        T += sig[s] * t
+    return T
+
+
+def test_hotrg2d_ynearest_two_point_equals_enumeration():
+    """<s s'> of y-neighbours on the 4x4 torus from the untruncated impurity
+    flow (trg/HOTRG_2d.py:300-348): exp(sum lnf_imp) Tr T_imp / Tr T."""
+    beta = 0.35
+    T, Ti = models.ising2d_tensor(beta), _spin_impurity(beta, 4)
+    Tn, Tin, lnf, lnf_imp = h2.ynearest_two_point_func_renorm(T, Ti, Ti, 16, 2, 2)
+    corr = np.exp(np.sum(lnf_imp)) * np.einsum("aabb", Tin) / np.einsum("aabb", Tn)
+    assert abs(corr - exact.brute_corr_ising(beta, (4, 4), (0, 0), (0, 1))) < 1e-12
+
+
+def test_hotrg3d_nearest_two_point_equals_enumeration():
+    """trg/HOTRG_3d.py:398-453 merges the two impurities in its first (Z)
+    step: z-neighbours on the 2x2x2 torus, untruncated.  The 3D flow
+    normalises by |Tr|, which is zero for a bare spin impurity, so the
+    impurity is 2 + s: <(2 + s)(2 + s')> = 4 + <s s'> (<s> = 0 at h = 0)."""
+    beta = 0.25
+    T, Ti = models.ising3d_tensor(beta), _spin_impurity(beta, 6, shift=2.0)
+    Tn, Tin, lnf, lnf_imp = h3.ynearest_two_point_func_renorm(T, Ti, Ti, 16, 1, 1, 1)
+    val = np.exp(np.sum(lnf_imp)) * np.einsum("xxyytt", Tin) / np.einsum("xxyytt", Tn)
+    assert abs(val - (4.0 + exact.brute_corr_ising(beta, (2, 2, 2), (0, 0, 0), (0, 0, 1)))) < 1e-12
+
+
+def test_hotrg2d_two_point_separated_equals_enumeration():
+    """Two impurities (nx, ny) = (1, 1) apart (trg/HOTRG_2d.py:219-297) on the 4x4 torus."""
+    beta = 0.35
+    T, Ti = models.ising2d_tensor(beta), _spin_impurity(beta, 4)
+    Tn, Tin, lnf, lnf_imp = h2.two_point_func_renorm(T, Ti, Ti, 1, 1, 16, 2, 2)
+    corr = np.exp(np.sum(lnf_imp)) * np.einsum("aabb", Tin) / np.einsum("aabb", Tn)
+    assert abs(corr - exact.brute_corr_ising(beta, (4, 4), (0, 0), (1, 1))) < 1e-12
+
+
+def test_hotrg3d_qr_one_point_equals_network_contraction():
+    """trg/HOTRG_3d_QR.py:859-908 untruncated on the 2x2x2 torus: the
+    one-point value P = Z(T0 at loc) / Z equals the direct contraction of the
+    eight-tensor network, for a random impurity at each of two sites (both
+    parities of the impurity coordinate in every direction)."""
+    import itertools
+    beta = 0.3
+    T = models.ising3d_tensor(beta)
+    T0 = T + 0.3 * np.random.default_rng(4).standard_normal(T.shape)
+    sites = list(itertools.product(range(2), range(2), range(2)))
+    Z = hq.network_value({s: T for s in sites}, (2, 2, 2))
+    for loc in [(0, 0, 0), (1, 0, 1)]:
+        Zi = hq.network_value({s: (T0 if s == loc else T) for s in sites}, (2, 2, 2))
+        for kind in ("QR", "original"):
+            _, _, P = hq.one_point_function({"Tensor": T, "loc": {}, "factor": {}},
+                                            {"Tensor": T0, "loc": {"X": loc[0], "Y": loc[1], "Z": loc[2]},
+                                             "factor": {}}, 16, {"X": 1, "Y": 1, "Z": 1}, kind)
+            assert abs(P - Zi / Z) < 1e-11 * abs(Zi / Z), (loc, kind, P, Zi / Z)

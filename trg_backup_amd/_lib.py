@@ -10,8 +10,8 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtrg.so")
 
-F32, F64, BF16, I32, BF16X3 = 0, 1, 2, 3, 4
-ABI = 4          # must match the "abi N" revision in trg_version()
+F32, F64, BF16, I32, BF16X3, BF16X6 = 0, 1, 2, 3, 4, 5
+ABI = 5          # must match the "abi N" revision in trg_version()
 
 c_int, c_ll, c_dbl, c_vp, c_ull, c_uint = (ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p,
                                            ctypes.c_ulonglong, ctypes.c_uint)

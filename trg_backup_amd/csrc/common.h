@@ -10,6 +10,11 @@ extern thread_local int g_last_hip_error;
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Entry points clear the thread's last HIP error before launching, so a stale
+// error left by an unrelated runtime call on this thread (which its caller
+// already saw) is not reported as this launch's failure.
+inline void clear_hip_error() { (void)hipGetLastError(); }
+
 // Launch-error check after a kernel launch: returns TRG_HIP_ERROR and records the code.
 inline int check_launch() {
     hipError_t e = hipGetLastError();

@@ -355,9 +355,33 @@ __device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
     lo = (__bf16)(x - (float)hi);
 }
 
+// SPLIT 2 ("bf16x6"): x = hi + mid + lo exactly (each residual is exact in
+// fp32 and the last one fits bf16), 24 significant bits like fp32 itself.
+// Products keep the six terms down to 2^-16 relative (lo lo, lo mid, mid lo
+// are dropped, below fp32 rounding), six bf16 MFMAs per block.
+__device__ inline void split3_bf16(float x, __bf16 &hi, __bf16 &mid, __bf16 &lo) {
+    hi = (__bf16)x;
+    const float r = x - (float)hi;
+    mid = (__bf16)r;
+    lo = (__bf16)(r - (float)mid);
+}
+
+// Stage one fp32 operand element as SPLIT + 1 bf16 pieces, piece p at S[p * stride].
+template <int SPLIT>
+__device__ inline void stage_pieces(float x, __bf16 *S, int stride) {
+    if constexpr (SPLIT == 0) {
+        S[0] = (__bf16)x;
+    } else if constexpr (SPLIT == 1) {
+        split_bf16(x, S[0], S[stride]);
+    } else {
+        split3_bf16(x, S[0], S[stride], S[2 * stride]);
+    }
+}
+
 // One BK slice of MFMAs from the k-fastest LDS tiles (hi parts at offset 0,
-// lo parts after BM * KP / BN * KP elements when SPLIT).
-template <int BM, int BN, int BK, int KP, int WM, int WN, bool SPLIT>
+// then lo (SPLIT 1) or mid and lo (SPLIT 2) parts, BM * KP / BN * KP
+// elements apart).  Small terms are accumulated before large ones.
+template <int BM, int BN, int BK, int KP, int WM, int WN, int SPLIT>
 __device__ inline void bf16_slice(const __bf16 *As, const __bf16 *Bs, f32x16 (&acc)[WM / 32][WN / 32], int wm,
                                   int wn, int lane) {
     constexpr int TM = WM / 32, TN = WN / 32;
@@ -365,26 +389,44 @@ __device__ inline void bf16_slice(const __bf16 *As, const __bf16 *Bs, f32x16 (&a
     for (int kk = 0; kk < BK; kk += 16) {
         bf16x8 a[TM], bb[TN];
         const int kr = kk + 8 * (lane >> 5);
+        auto fa = [&](int piece, int i) {
+            return *reinterpret_cast<const bf16x8 *>(&As[piece * BM * KP + (wm * WM + i * 32 + (lane & 31)) * KP + kr]);
+        };
+        auto fb = [&](int piece, int j) {
+            return *reinterpret_cast<const bf16x8 *>(&Bs[piece * BN * KP + (wn * WN + j * 32 + (lane & 31)) * KP + kr]);
+        };
 #pragma unroll
-        for (int i = 0; i < TM; ++i)
-            a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm * WM + i * 32 + (lane & 31)) * KP + kr]);
+        for (int i = 0; i < TM; ++i) a[i] = fa(0, i);
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
-            bb[j] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn * WN + j * 32 + (lane & 31)) * KP + kr]);
-        if constexpr (SPLIT) {
+        for (int j = 0; j < TN; ++j) bb[j] = fb(0, j);
+        if constexpr (SPLIT == 1) {
             bf16x8 al[TM], bl[TN];
 #pragma unroll
-            for (int i = 0; i < TM; ++i)
-                al[i] = *reinterpret_cast<const bf16x8 *>(&As[BM * KP + (wm * WM + i * 32 + (lane & 31)) * KP + kr]);
+            for (int i = 0; i < TM; ++i) al[i] = fa(1, i);
 #pragma unroll
-            for (int j = 0; j < TN; ++j)
-                bl[j] = *reinterpret_cast<const bf16x8 *>(&Bs[BN * KP + (wn * WN + j * 32 + (lane & 31)) * KP + kr]);
+            for (int j = 0; j < TN; ++j) bl[j] = fb(1, j);
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int j = 0; j < TN; ++j) {
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bb[j], acc[i][j], 0, 0, 0);
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bl[j], acc[i][j], 0, 0, 0);
+                }
+        } else if constexpr (SPLIT == 2) {
+            bf16x8 am[TM], bm[TN], al[TM], bl[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) { am[i] = fa(1, i); al[i] = fa(2, i); }
+#pragma unroll
+            for (int j = 0; j < TN; ++j) { bm[j] = fb(1, j); bl[j] = fb(2, j); }
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bb[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[i], bm[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[i], bb[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bm[j], acc[i][j], 0, 0, 0);
                 }
         }
 #pragma unroll
@@ -395,7 +437,7 @@ __device__ inline void bf16_slice(const __bf16 *As, const __bf16 *Bs, f32x16 (&a
     }
 }
 
-template <int BM, int BN, int BK, int WM, int WN, bool A_MCONT, bool B_NCONT, bool SPLIT, typename TC>
+template <int BM, int BN, int BK, int WM, int WN, bool A_MCONT, bool B_NCONT, int SPLIT, typename TC>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
 gemm_bf16_kernel(GemmArgs g) {
     constexpr int WAVES_M = BM / WM;
@@ -405,7 +447,7 @@ gemm_bf16_kernel(GemmArgs g) {
     constexpr int KP = BK + 8;  // row pitch in bf16 elements (16-byte aligned rows)
     constexpr int AE = BM * BK / NT;
     constexpr int BE = BN * BK / NT;
-    constexpr int NS = SPLIT ? 2 : 1;
+    constexpr int NS = SPLIT + 1;
     static_assert(BK % 16 == 0, "BK must be a multiple of 16");
 
     __shared__ __attribute__((aligned(16))) __bf16 As[NS * BM * KP];
@@ -455,16 +497,14 @@ gemm_bf16_kernel(GemmArgs g) {
             const int e = tid + i * NT;
             int mm, kk;
             if (A_MCONT) { kk = e / BM; mm = e % BM; } else { mm = e / BK; kk = e % BK; }
-            if constexpr (SPLIT) split_bf16(ra[i], As[mm * KP + kk], As[BM * KP + mm * KP + kk]);
-            else As[mm * KP + kk] = (__bf16)ra[i];
+            stage_pieces<SPLIT>(ra[i], As + mm * KP + kk, BM * KP);
         }
 #pragma unroll
         for (int i = 0; i < BE; ++i) {
             const int e = tid + i * NT;
             int nn, kk;
             if (B_NCONT) { kk = e / BN; nn = e % BN; } else { nn = e / BK; kk = e % BK; }
-            if constexpr (SPLIT) split_bf16(rb[i], Bs[nn * KP + kk], Bs[BN * KP + nn * KP + kk]);
-            else Bs[nn * KP + kk] = (__bf16)rb[i];
+            stage_pieces<SPLIT>(rb[i], Bs + nn * KP + kk, BN * KP);
         }
     };
 
@@ -501,7 +541,7 @@ gemm_bf16_kernel(GemmArgs g) {
 
 // DB: two LDS stages; the next slice is written into the stage the waves are
 // not reading, so one barrier per K slice suffices instead of two.
-template <int BM, int BN, int BK, int WM, int WN, bool SPLIT, bool DB, typename TC>
+template <int BM, int BN, int BK, int WM, int WN, int SPLIT, bool DB, typename TC>
 __global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
 gemm_bf16k_kernel(GemmArgs g) {
     constexpr int WAVES_M = BM / WM;
@@ -512,7 +552,7 @@ gemm_bf16k_kernel(GemmArgs g) {
     constexpr int KQ = BK / 4;            // fp32 vectors per tile row
     constexpr int AV = BM * KQ / NT;
     constexpr int BV = BN * KQ / NT;
-    constexpr int NS = SPLIT ? 2 : 1;
+    constexpr int NS = SPLIT + 1;
     static_assert(AV * NT == BM * KQ && BV * NT == BN * KQ, "tile not divisible by threads");
     static_assert(BK % 16 == 0, "BK must be a multiple of 16");
 
@@ -558,17 +598,23 @@ gemm_bf16k_kernel(GemmArgs g) {
         }
     };
     auto put = [&](__bf16 *S, int rows, int off, const f32x4 &x) {
-        bf16x4 hi, lo;
+        bf16x4 hi, mid, lo;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            __bf16 h, l;
-            if constexpr (SPLIT) split_bf16(x[t], h, l);
+            __bf16 h, m, l;
+            if constexpr (SPLIT == 2) split3_bf16(x[t], h, m, l);
+            else if constexpr (SPLIT == 1) split_bf16(x[t], h, l);
             else h = (__bf16)x[t];
             hi[t] = h;
-            if constexpr (SPLIT) lo[t] = l;
+            if constexpr (SPLIT == 2) mid[t] = m;
+            if constexpr (SPLIT >= 1) lo[t] = l;
         }
         *reinterpret_cast<bf16x4 *>(&S[off]) = hi;
-        if constexpr (SPLIT) *reinterpret_cast<bf16x4 *>(&S[rows * KP + off]) = lo;
+        if constexpr (SPLIT == 1) *reinterpret_cast<bf16x4 *>(&S[rows * KP + off]) = lo;
+        if constexpr (SPLIT == 2) {
+            *reinterpret_cast<bf16x4 *>(&S[rows * KP + off]) = mid;
+            *reinterpret_cast<bf16x4 *>(&S[2 * rows * KP + off]) = lo;
+        }
     };
     auto store = [&](int stage) {
 #pragma unroll
@@ -768,14 +814,14 @@ static bool kvec_ok(const GemmArgs &g, bool amc, bool bnc) {
 // KONLY: the tile exists for the K-contiguous kernel only (its generic
 // counterpart would spill registers); other layouts run tile 256x128x32 w64x64
 // (128x128x32 w64x64 when SPLIT), which gives the same bits.
-template <int BM, int BN, int BK, int WM, int WN, bool SPLIT, bool KONLY = false, bool DB = false,
+template <int BM, int BN, int BK, int WM, int WN, int SPLIT, bool KONLY = false, bool DB = false,
           typename TC = float>
 static int launch_bf16(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
     static_assert(!DB || KONLY, "double-buffered tiles exist for the K-contiguous kernel only");
     if constexpr (KONLY) {
         if (!kvec_ok(g, amc, bnc)) {
-            if constexpr (SPLIT) return launch_bf16<128, 128, 32, 64, 64, true, false, false, TC>(g, batch, amc, bnc, s);
-            else return launch_bf16<256, 128, 32, 64, 64, false, false, false, TC>(g, batch, amc, bnc, s);
+            if constexpr (SPLIT > 0) return launch_bf16<128, 128, 32, 64, 64, SPLIT, false, false, TC>(g, batch, amc, bnc, s);
+            else return launch_bf16<256, 128, 32, 64, 64, 0, false, false, TC>(g, batch, amc, bnc, s);
         }
     }
     g.tiles_m = (int)cdiv(g.M, BM);
@@ -903,6 +949,18 @@ static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmAr
             default: return TRG_BAD_ARG;
         }
     }
+    if (dtype == TRG_F32 && compute == TRG_BF16X6) {
+        if (config < 0) config = work >= 256LL * 128 * 128 ? 0 : 1;
+        switch (config) {
+            case 0: return launch_bf16<128, 128, 32, 64, 64, 2>(g, batch, amc, bnc, s);
+            case 1: return launch_bf16<64, 64, 32, 32, 32, 2>(g, batch, amc, bnc, s);
+            case 2: return launch_bf16<256, 128, 32, 64, 64, 2>(g, batch, amc, bnc, s);
+            case 3: return launch_bf16<128, 256, 32, 64, 64, 2>(g, batch, amc, bnc, s);
+            case 4: return launch_bf16<256, 256, 32, 128, 64, 2, true>(g, batch, amc, bnc, s);
+            case 5: return launch_bf16<256, 128, 32, 128, 64, 2, true>(g, batch, amc, bnc, s);
+            default: return TRG_BAD_ARG;
+        }
+    }
     if (dtype == TRG_F64 && compute == TRG_F64) {
         if (config < 0) config = work >= 256LL * 128 * 128 ? 0 : 1;
         switch (config) {
@@ -927,6 +985,7 @@ extern "C" int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, 
                         const void *B, long long sBb, long long rsB, long long csB,
                         double beta, void *C, long long sCb, long long rsC, long long csC,
                         void *stream) {
+    clear_hip_error();
     if (batch < 0 || M < 0 || N < 0 || K < 0) return TRG_BAD_ARG;
     if (batch == 0 || M == 0 || N == 0) return TRG_OK;
     if (!A || !B || !C) return TRG_BAD_ARG;
@@ -952,6 +1011,7 @@ extern "C" int trg_gemm_config_count(int dtype, int c_dtype, int compute) {
     if (dtype == TRG_F32 && compute == TRG_F32) return 8;
     if (dtype == TRG_F32 && compute == TRG_BF16) return 12;
     if (dtype == TRG_F32 && compute == TRG_BF16X3) return 8;
+    if (dtype == TRG_F32 && compute == TRG_BF16X6) return 6;
     if (dtype == TRG_F64 && compute == TRG_F64) return 4;
     return 0;
 }
@@ -970,6 +1030,8 @@ extern "C" const char *trg_gemm_config_name(int dtype, int c_dtype, int compute,
                                 "256x256x64 w128x64", "256x256x32 w128x64 db", "256x128x64 w64x64 db",
                                 "128x256x32 w64x64", "256x256x64 w128x64 db", "256x256x128 w128x64",
                                 "128x256x64 w64x64 db"};
+    static const char *bx6[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "128x256x32 w64x64",
+                                "256x256x32 w128x64", "256x128x32 w128x64"};
     static const char *f64[] = {"128x128x16 w64x64", "64x64x16 w32x32", "128x64x16 w64x32", "64x128x16 w32x64"};
     if (config < 0 || config >= trg_gemm_config_count(dtype, c_dtype, compute)) return "";
     if (dtype == TRG_BF16) return bin[config];
@@ -977,6 +1039,7 @@ extern "C" const char *trg_gemm_config_name(int dtype, int c_dtype, int compute,
     if (dtype == TRG_F32 && compute == TRG_F32) return f32[config];
     if (dtype == TRG_F32 && compute == TRG_BF16) return bf[config];
     if (dtype == TRG_F32 && compute == TRG_BF16X3) return bx3[config];
+    if (dtype == TRG_F32 && compute == TRG_BF16X6) return bx6[config];
     return f64[config];
 }
 
@@ -985,6 +1048,7 @@ extern "C" int trg_gemm_ex(int config, int dtype, int c_dtype, int compute, int 
                            const void *B, long long sBb, long long rsB, long long csB,
                            double beta, void *C, long long sCb, long long rsC, long long csC,
                            int cdm, long long csm, int cdn, long long csn, void *stream) {
+    clear_hip_error();
     if (batch < 0 || M < 0 || N < 0 || K < 0 || cdm < 0 || cdn < 0) return TRG_BAD_ARG;
     if (batch == 0 || M == 0 || N == 0) return TRG_OK;
     if (!A || !B || !C) return TRG_BAD_ARG;

@@ -325,6 +325,7 @@ using namespace trg;
 
 extern "C" int trg_syevj_f64(int batch, int n, double *A, long long sA, double *w, double *V, int max_sweeps,
                              int *info, void *stream) {
+    clear_hip_error();
     if (batch < 0 || n < 0 || max_sweeps < 1) return TRG_BAD_ARG;
     if (batch == 0 || n == 0) return TRG_OK;
     if (n > JMAX) return TRG_SHAPE;
@@ -345,6 +346,7 @@ extern "C" int trg_syevj_f64(int batch, int n, double *A, long long sA, double *
 
 extern "C" int trg_chol_inv_f64(int batch, int l, double *G, long long sG, double shift_rel, double *Rinv, int *info,
                                 void *stream) {
+    clear_hip_error();
     if (batch < 0 || l < 0) return TRG_BAD_ARG;
     if (batch == 0 || l == 0) return TRG_OK;
     if (l > 256) return TRG_SHAPE;
@@ -445,6 +447,7 @@ static int cholqr3(int batch, int n, int l, double *X, double *Y, double *G, dou
 extern "C" int trg_eigh_topk_f64(int batch, int n, int k, int l, int q, const double *A, long long sA, double *w,
                                  double *V, double *eps, unsigned long long seed, void *ws, long long ws_bytes,
                                  int *info, void *stream) {
+    clear_hip_error();
     if (batch < 0 || n < 0 || k < 0 || l < 0 || q < 0) return TRG_BAD_ARG;
     if (batch == 0 || n == 0 || k == 0) return TRG_OK;
     if (k > l || l > n || l > 256) return TRG_SHAPE;
@@ -664,6 +667,7 @@ __global__ void truncate_kernel(int n, const double *s, int k, double deg_eps, d
 
 extern "C" int trg_gesvj_f64(int batch, int m, int n, const double *A, long long sA, double *s, double *U, double *V,
                              int max_sweeps, int *info, void *stream) {
+    clear_hip_error();
     if (batch < 0 || m < 0 || n < 0 || max_sweeps < 1) return TRG_BAD_ARG;
     if (batch == 0 || m == 0 || n == 0) return TRG_OK;
     if (n > m || m > JMAX) return TRG_SHAPE;
@@ -683,6 +687,7 @@ extern "C" int trg_gesvj_f64(int batch, int m, int n, const double *A, long long
 
 extern "C" int trg_truncate_rank(int n, const double *s, int k, double degeneracy_eps, double truncate_eps, int *chi,
                                  double *err, void *stream) {
+    clear_hip_error();
     if (n < 0 || k < 0) return TRG_BAD_ARG;
     if (!s || !chi || !err) return TRG_BAD_ARG;
     if (n == 0 || n > TRUNC_MAX) return TRG_SHAPE;
@@ -692,6 +697,7 @@ extern "C" int trg_truncate_rank(int n, const double *s, int k, double degenerac
 
 extern "C" int trg_potrf_psd_f64(int batch, int l, double *G, long long sG, double psd_tol, const double *scale,
                                  double *R, double *Rinv, void *stream) {
+    clear_hip_error();
     if (batch < 0 || l < 0 || psd_tol <= 0) return TRG_BAD_ARG;
     if (batch == 0 || l == 0) return TRG_OK;
     if (l > 256) return TRG_SHAPE;
@@ -719,6 +725,7 @@ extern "C" long long trg_orthonormalize_workspace(int batch, int m, int l) {
 
 extern "C" int trg_orthonormalize_f64(int batch, int m, int l, double *X, long long sX, double *Q, void *ws,
                                       long long ws_bytes, int passes, int *info, void *stream) {
+    clear_hip_error();
     // X (batch x m x l, stride sX) is clobbered; the orthonormal basis of its
     // column span lands in Q (stride m*l).  Shifted CholeskyQR3 (passes = 3)
     // or its shifted first pass alone (passes = 1).
@@ -746,6 +753,7 @@ extern "C" int trg_orthonormalize_f64(int batch, int m, int l, double *X, long l
 // Y[b] += alpha_rel * scal[b] * X[b]
 extern "C" int trg_axpy_dev_f64(int batch, long long n, double alpha_rel, const double *scal, const double *X,
                                 long long sX, double *Y, long long sY, void *stream) {
+    clear_hip_error();
     if (batch < 0 || n < 0) return TRG_BAD_ARG;
     if (batch == 0 || n == 0) return TRG_OK;
     if (!scal || !X || !Y) return TRG_BAD_ARG;

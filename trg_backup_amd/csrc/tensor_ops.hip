@@ -324,6 +324,7 @@ extern "C" int trg_last_hip_error(void) { return g_last_hip_error; }
 
 extern "C" int trg_copy_strided(int dtype_in, int dtype_out, int rank, const long long *dims, const void *in,
                                 const long long *si, void *out, const long long *so, void *stream) {
+    clear_hip_error();
     if (rank < 0 || rank > 8 || !dims || !si || !so) return TRG_BAD_ARG;
     StridedDesc d;
     d.rank = rank;
@@ -423,6 +424,7 @@ extern "C" int trg_copy_strided(int dtype_in, int dtype_out, int rank, const lon
 }
 
 extern "C" int trg_fill(int dtype, long long n, double value, void *out, void *stream) {
+    clear_hip_error();
     if (n < 0) return TRG_BAD_ARG;
     if (n == 0) return TRG_OK;
     if (!out) return TRG_BAD_ARG;
@@ -436,6 +438,7 @@ extern "C" int trg_fill(int dtype, long long n, double value, void *out, void *s
 
 extern "C" int trg_maxabs(int dtype, int batch, long long n, const void *T, long long sb, double *c,
                           double *part, int nchunk, void *stream) {
+    clear_hip_error();
     if (batch < 0 || n < 0 || nchunk < 1) return TRG_BAD_ARG;
     if (batch == 0) return TRG_OK;
     if (!T || !c || !part) return TRG_BAD_ARG;
@@ -451,6 +454,7 @@ extern "C" int trg_maxabs(int dtype, int batch, long long n, const void *T, long
 
 extern "C" int trg_trace(int dtype, int batch, int rank, const long long *dims, const long long *strides,
                          const void *T, long long sb, double *t, void *stream) {
+    clear_hip_error();
     if (batch < 0 || rank < 1 || rank > 4 || !dims || !strides) return TRG_BAD_ARG;
     if (batch == 0) return TRG_OK;
     if (!T || !t) return TRG_BAD_ARG;
@@ -469,6 +473,7 @@ extern "C" int trg_trace(int dtype, int batch, int rank, const long long *dims, 
 
 extern "C" int trg_normalize(int dtype, int batch, long long n, void *T, long long sb, const double *c_in,
                              double *lnf, long long lnf_stride, int step, int *step_ptr, void *stream) {
+    clear_hip_error();
     if (batch < 0 || n < 0 || step < 0) return TRG_BAD_ARG;
     if (batch == 0) return TRG_OK;
     if (!c_in || (n > 0 && !T)) return TRG_BAD_ARG;
@@ -494,6 +499,7 @@ extern "C" int trg_normalize(int dtype, int batch, long long n, void *T, long lo
 
 extern "C" int trg_scale_rows(int dtype, int batch, long long n, long long period, long long inner, void *Y,
                               long long sY, const void *x, long long sx, int mode, void *stream) {
+    clear_hip_error();
     if (batch < 0 || n < 0 || period < 1 || inner < 1 || mode < 0 || mode > 3) return TRG_BAD_ARG;
     if (batch == 0 || n == 0) return TRG_OK;
     if (!Y || !x) return TRG_BAD_ARG;
@@ -508,6 +514,7 @@ extern "C" int trg_scale_rows(int dtype, int batch, long long n, long long perio
 
 extern "C" int trg_select_isometry(int dtype, int batch, long long nk, const double *eps0, const double *eps1,
                                    const double *V0, const double *V1, void *U, int *flag, void *stream) {
+    clear_hip_error();
     if (batch < 0 || nk < 0) return TRG_BAD_ARG;
     if (batch == 0) return TRG_OK;
     if (!eps0 || !eps1 || !V0 || !V1 || !U) return TRG_BAD_ARG;
@@ -523,6 +530,7 @@ extern "C" int trg_select_isometry(int dtype, int batch, long long nk, const dou
 extern "C" int trg_gaussian(int dtype, int batch, long long n, unsigned long long seed, unsigned int stream_id,
                             unsigned int stream_stride, unsigned long long offset, void *out, long long sb,
                             void *stream) {
+    clear_hip_error();
     if (batch < 0 || n < 0) return TRG_BAD_ARG;
     if (batch == 0 || n == 0) return TRG_OK;
     if (!out) return TRG_BAD_ARG;
@@ -537,6 +545,7 @@ extern "C" int trg_gaussian(int dtype, int batch, long long n, unsigned long lon
 
 extern "C" int trg_lnz(int batch, int nterms, const double *lnf, long long lnf_stride, const double *tr,
                        double inv_volume, double *out, void *stream) {
+    clear_hip_error();
     if (batch < 0 || nterms < 0) return TRG_BAD_ARG;
     if (batch == 0) return TRG_OK;
     if (!lnf || !out) return TRG_BAD_ARG;

@@ -31,10 +31,14 @@ INSTANCE_BUDGET = 1 << 30
 MERGE_BUDGET = 32 << 30
 
 
+SPLIT_COMPUTE = ("bf16", "bf16x3", "bf16x6")   # GEMM compute types on the bf16 MFMA
+
+
 @dataclass(frozen=True)
 class Precision:
     """Storage dtype of T and the GEMM compute type of the environment and
-    the merge ("f64", "f32", "bf16x3" = split-bf16 products with about 16
+    the merge ("f64", "f32", "bf16x6" = three-piece split-bf16 products with
+    the 24 bits of fp32, "bf16x3" = split-bf16 products with about 16
     significant bits per operand, "bf16").  Eigensolvers, traces and
     log-norms are always float64."""
     store: torch.dtype = torch.float64
@@ -42,10 +46,10 @@ class Precision:
     merge: str = "f64"
 
     def env_compute(self):
-        return None if self.store == torch.float64 else (self.env if self.env in ("bf16", "bf16x3") else None)
+        return None if self.store == torch.float64 else (self.env if self.env in SPLIT_COMPUTE else None)
 
     def merge_compute(self):
-        return None if self.store == torch.float64 else (self.merge if self.merge in ("bf16", "bf16x3") else None)
+        return None if self.store == torch.float64 else (self.merge if self.merge in SPLIT_COMPUTE else None)
 
 
 FP64 = Precision()
@@ -59,12 +63,13 @@ class Policy:
     early steps run in float64 and later ones in lower precision."""
 
     def __init__(self, fp64_steps=10**9, low=FP32, bf16_env_from=None, bf16_merge_from=None,
-                 split_merge_from=None):
+                 split_merge_from=None, split6_merge_from=None):
         self.fp64_steps = fp64_steps
         self.low = low
         self.bf16_env_from = bf16_env_from
         self.bf16_merge_from = bf16_merge_from
         self.split_merge_from = split_merge_from
+        self.split6_merge_from = split6_merge_from
 
     def __call__(self, step):
         if step <= self.fp64_steps:
@@ -72,6 +77,8 @@ class Policy:
         env, merge = self.low.env, self.low.merge
         if self.bf16_env_from is not None and step > self.bf16_env_from:
             env = "bf16"
+        if self.split6_merge_from is not None and step > self.split6_merge_from:
+            merge = "bf16x6"
         if self.split_merge_from is not None and step > self.split_merge_from:
             merge = "bf16x3"
         if self.bf16_merge_from is not None and step > self.bf16_merge_from:
@@ -84,6 +91,8 @@ class Policy:
         s = f"fp64 for steps <= {self.fp64_steps}, then {self.low.store} env={self.low.env} merge={self.low.merge}"
         if self.bf16_env_from is not None:
             s += f", bf16 env after step {self.bf16_env_from}"
+        if self.split6_merge_from is not None:
+            s += f", bf16x6 merge after step {self.split6_merge_from}"
         if self.split_merge_from is not None:
             s += f", bf16x3 merge after step {self.split_merge_from}"
         if self.bf16_merge_from is not None:
@@ -197,7 +206,7 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
     # so no chi^5 intermediate is permuted; the column order of each product
     # keeps its innermost index contiguous in the destination (coalesced
     # epilogue stores).
-    kinner = compute in ("bf16", "bf16x3")
+    kinner = compute in SPLIT_COMPUTE
     # plain bf16: the two operands of the chi^7 GEMM are stored in bf16 (the
     # rounding the MFMA would apply anyway, done once by the producer), which
     # halves the operand traffic of the L2-fed bf16 kernel; same bits

@@ -8,7 +8,7 @@ import math
 import torch
 
 from . import _lib, autotune
-from ._lib import BF16, BF16X3, F32, F64, I32, call, ll_array
+from ._lib import BF16, BF16X3, BF16X6, F32, F64, I32, call, ll_array
 
 _DT = {torch.float32: F32, torch.float64: F64, torch.bfloat16: BF16}
 
@@ -136,7 +136,7 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_layout=N
 
     A is (M, K) or (batch, M, K), B is (K, N) or (batch, K, N).  Views may
     be transposed as long as one of the two matrix strides of A and of B is 1.
-    compute: None (same as storage), "bf16" / "bf16x3" (f32 storage only) or a dtype code.
+    compute: None (same as storage), "bf16" / "bf16x3" / "bf16x6" (f32 storage only) or a dtype code.
     out_layout: write the product into a permuted tensor instead of a matrix
     view: C is then the destination tensor (contiguous, batch-major) and
     out_layout = (rows, cols), each a list of (size, stride) from outer to
@@ -196,6 +196,8 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_layout=N
         cc = BF16
     elif compute == "bf16x3":
         cc = BF16X3
+    elif compute == "bf16x6":
+        cc = BF16X6
     elif compute == "f32":
         cc = F32
     elif compute == "f64":

@@ -8,8 +8,9 @@ HOTRG_SQUEEZER (default "QR"), `degeneracy_eps` from DGENERACY_EPS (default
 which only splits the reference's contraction loop and does not change the
 result.  The diagnostics written on every step (cal_X, cal_coorelation,
 cal_scaling_dimensions, prints and timings) are not emitted.  The squeezers
-take the pure-tensor form (T0 = T1 = T2 = T3); the impurity flows
-(:428-557, :798-1033) are not on the device.
+take the pure-tensor form (T0 = T1 = T2 = T3); of the impurity flows the
+one-point function (:520-557, :798-908) is on the device, the two-point
+flows (:428-518, :910-1033) and the diagnostics are not.
 """
 import math
 import os
@@ -166,6 +167,90 @@ def ln_Z_over_V(T, Dcut, TOT_RGSTEPS):
                                       PRECISION)[0].item())
 
 
+def _prec_of(Td):
+    return hotrg.FP64 if Td.dtype == torch.float64 else hotrg.FP32
+
+
+def _trace_norm(Td):
+    """(T / |Tr T|, |Tr T|) on the device, Tr over (xx)(yy)(zz)."""
+    Tb = ops.contiguous(Td[None])
+    c = ops.trace_pairs(Tb)
+    ops.normalize_(Tb, c)
+    return Tb[0], abs(float(c[0].item()))
+
+
+def _impure_step(Td, T0d, odd, Dcut, direction):
+    """One step of the pure tensor and one impurity (trg/HOTRG_3d_QR.py:520-557):
+    projectors from the pure tensor; the impurity is the upper tensor of its
+    pair when `odd`, the lower one otherwise."""
+    prec = _prec_of(Td)
+    Tb = hotrg.rearrange3d(ops.contiguous(Td[None]), direction)
+    T0b = hotrg.rearrange3d(ops.contiguous(T0d[None]), direction)
+    (_, U1a, U1b, U2a, U2b), = hotrg_qr.squeeze(Tb, Dcut, select_squeezer, degeneracy_eps, prec)
+    lo, up = (Tb, T0b) if odd else (T0b, Tb)
+    T0n = hotrg.merge3d(lo, up, U1a, U1b, U2a, U2b, compute=prec.merge_compute())
+    Tn = hotrg.merge3d(Tb, Tb, U1a, U1b, U2a, U2b, compute=prec.merge_compute())
+    return hotrg.restore3d(Tn, direction)[0], hotrg.restore3d(T0n, direction)[0]
+
+
+def new_impuer_tensor_1point(T, T0, Dcut, direction):
+    """trg/HOTRG_3d_QR.py:520-557 on the dicts (T0['loc'] gives the impurity's coordinates)."""
+    (Td, kind), (T0d, _) = to_device(T["Tensor"]), to_device(T0["Tensor"])
+    Tn, T0n = _impure_step(Td, T0d, T0["loc"][direction] % 2 == 1, Dcut, direction)
+    T["Tensor"], T0["Tensor"] = from_device(Tn, kind), from_device(T0n, kind)
+    return T, T0
+
+
+def exec_rgstep_1point_func(T, T0, Dcut, direction, rgstep):
+    """trg/HOTRG_3d_QR.py:833-857: step, normalise both tensors, store
+    ln c / 2^n for T and ln c0 - ln c for T0, halve the impurity coordinate."""
+    key = f"({rgstep['X']},{rgstep['Y']},{rgstep['Z']})"
+    (Td, kind), (T0d, _) = to_device(T["Tensor"]), to_device(T0["Tensor"])
+    Tn, T0n = _impure_step(Td, T0d, T0["loc"][direction] % 2 == 1, Dcut, direction)
+    Tn, c = _trace_norm(Tn)
+    T0n, c0 = _trace_norm(T0n)
+    T["Tensor"], T0["Tensor"] = from_device(Tn, kind), from_device(T0n, kind)
+    T["factor"][key] = math.log(c) / 2 ** sum(rgstep.values())
+    T0["factor"][key] = math.log(c0) - math.log(c)
+    T0["loc"][direction] //= 2
+    return T, T0
+
+
+def cal_field_expected_value(T, T0, rgstep, save):
+    """trg/HOTRG_3d_QR.py:798-831: returns (ln Z/V, [P_i]) with
+    P_i = exp(sum T0[i] factors) Tr T0[i] / Tr T; with save and OUTPUT_DIR set,
+    appends the reference's line to field_expected_value_rgsteps_dependence.dat."""
+    def tr(x):
+        Xd, _ = to_device(x)
+        return float(ops.trace_pairs(ops.contiguous(Xd[None]))[0].item())
+    TrT = tr(T["Tensor"])
+    P = [math.exp(sum(t["factor"].values())) * tr(t["Tensor"]) / TrT for t in T0]
+    n = sum(rgstep.values())
+    lnz = sum(T["factor"].values()) + math.log(TrT) / 2 ** n
+    out = os.environ.get("OUTPUT_DIR")
+    if save and out:
+        with open(os.path.join(out, "field_expected_value_rgsteps_dependence.dat"), "w" if n == 0 else "a") as f:
+            f.write(f"{n} {lnz:.12e} {P[0]:.12e} {0.0}\n")
+    return lnz, P
+
+
+def one_point_function(T, T0, Dcut, TOT_RGSTEPS):
+    """trg/HOTRG_3d_QR.py:859-908: the pure tensor and one impurity through
+    the "ZXY" cycle; returns (T, T0) with their factor dicts filled.  The
+    one-point value is cal_field_expected_value(T, [T0], ...)[1][0]."""
+    rgstep = {"X": 0, "Y": 0, "Z": 0}
+    (Td, kind), (T0d, _) = to_device(T["Tensor"]), to_device(T0["Tensor"])
+    Td, c = _trace_norm(Td)
+    T0d, c0 = _trace_norm(T0d)
+    T["Tensor"], T0["Tensor"] = from_device(Td, kind), from_device(T0d, kind)
+    T["factor"]["(0,0,0)"] = math.log(c)
+    T0["factor"]["(0,0,0)"] = math.log(c0) - math.log(c)
+    for direction in hotrg_qr.schedule(TOT_RGSTEPS):
+        rgstep[direction] += 1
+        T, T0 = exec_rgstep_1point_func(T, T0, Dcut, direction, rgstep)
+    return T, T0
+
+
 def _not_on_device(name, lines):
     def f(*args, **kwargs):
         raise NotImplementedError(f"HOTRG_3d_QR.{name} (trg/HOTRG_3d_QR.py:{lines}) is not on the device")
@@ -175,13 +260,9 @@ def _not_on_device(name, lines):
 
 new_impuer_tensor_2points = _not_on_device("new_impuer_tensor_2points", "428-475")
 new_impuer_tensor_absorb = _not_on_device("new_impuer_tensor_absorb", "477-518")
-new_impuer_tensor_1point = _not_on_device("new_impuer_tensor_1point", "520-557")
 cal_X = _not_on_device("cal_X", "559-583")
 find_1stto3rdmax = _not_on_device("find_1stto3rdmax", "585-611")
 cal_scaling_dimensions = _not_on_device("cal_scaling_dimensions", "613-637")
 cal_coorelation = _not_on_device("cal_coorelation", "639-713")
-cal_field_expected_value = _not_on_device("cal_field_expected_value", "798-831")
-exec_rgstep_1point_func = _not_on_device("exec_rgstep_1point_func", "833-857")
-one_point_function = _not_on_device("one_point_function", "859-908")
 exec_rgstep_2point_func = _not_on_device("exec_rgstep_2point_func", "910-995")
 two_point_function = _not_on_device("two_point_function", "997-1033")
