@@ -92,6 +92,7 @@ class Server:
         self.atrg_q = atrg_q
         self.atrg_deg_eps = atrg_deg_eps
         self.streams = [torch.cuda.Stream() for _ in range(streams)]   # default priority
+        self.setup_stream = torch.cuda.Stream()                       # flow set-up (main thread)
         self.pool = ThreadPoolExecutor(max_workers=streams)
         self.lock = threading.Lock()
         self.rejected = []
@@ -99,6 +100,10 @@ class Server:
 
     # ---- flow set-up and stepping ----------------------------------------
     def _start(self, fl):
+        with torch.cuda.stream(self.setup_stream):
+            self._start_on_stream(fl)
+
+    def _start_on_stream(self, fl):
         r = fl.req
         if fl.scheme == "hotrg2d":
             T = models.initial_tensor(r.model, r.beta)[None].contiguous()
@@ -229,8 +234,8 @@ class Server:
                 admitted = True
                 qi += 1
             if admitted:
-                # set-up kernels ran on this thread's stream; the workers' streams must see them done
-                torch.cuda.current_stream().synchronize()
+                # set-up kernels ran on the set-up stream; the workers' streams must see them done
+                self.setup_stream.synchronize()
             while free and ready:
                 group = self._take_group(ready)
                 si = free.pop(0)
