@@ -16,6 +16,9 @@
 // instead of separate copies whenever the contiguous leg allows it.
 #include "common.h"
 
+#include <type_traits>
+#include <utility>
+
 namespace trg {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -759,6 +762,186 @@ gemm_bf16in_kernel(GemmArgs g) {
 }
 
 // ---------------------------------------------------------------------------
+// Software-pipelined K-contiguous kernel: fp32 storage staged as SPLIT + 1
+// bf16 pieces, or bf16 storage (BIN) moved as is.  Two LDS stages and one
+// register set.  Iteration t first converts and writes slice t + 1 (loaded
+// into registers one iteration earlier) into the stage nobody reads, then
+// issues the global loads of slice t + 2, then runs the MFMAs of slice t from
+// the other stage, then one barrier: the global latency is hidden behind a
+// whole iteration of MFMAs.  Loads are unconditional (indices clamped into the
+// operand, out-of-range vectors replaced by zeros afterwards) and the last
+// slice is peeled off, so the loop body is a single basic block that the
+// compiler can interleave; SCHED = 1 also asks its instruction-group
+// scheduler for an explicit pattern (LDS reads of the first k step, then
+// each MFMA followed by a share of the LDS reads, conversion VALU work, LDS
+// writes and, halfway, the global loads).  Slices past the end are clamped
+// to the last one (loaded again, never used).  Same MFMA instruction, k order
+// and term order as gemm_bf16k_kernel and gemm_bf16in_kernel, so the results
+// agree bit for bit.
+// ---------------------------------------------------------------------------
+template <int SPLIT>
+__device__ inline void put_pieces(__bf16 *S, int piece_stride, int off, const f32x4 &x) {
+    bf16x4 hi, mid, lo;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        __bf16 h, m, l;
+        if constexpr (SPLIT == 2) split3_bf16(x[t], h, m, l);
+        else if constexpr (SPLIT == 1) split_bf16(x[t], h, l);
+        else h = (__bf16)x[t];
+        hi[t] = h;
+        if constexpr (SPLIT == 2) mid[t] = m;
+        if constexpr (SPLIT >= 1) lo[t] = l;
+    }
+    *reinterpret_cast<bf16x4 *>(&S[off]) = hi;
+    if constexpr (SPLIT == 1) *reinterpret_cast<bf16x4 *>(&S[piece_stride + off]) = lo;
+    if constexpr (SPLIT == 2) {
+        *reinterpret_cast<bf16x4 *>(&S[piece_stride + off]) = mid;
+        *reinterpret_cast<bf16x4 *>(&S[2 * piece_stride + off]) = lo;
+    }
+}
+
+// One slot of the interleave pattern: an MFMA followed by its share of the
+// other instruction streams (sched_group_barrier masks: 0x008 MFMA, 0x100 LDS
+// read, 0x002 VALU, 0x200 LDS write, 0x020 global read).
+template <int DSR, int VA, int DSW, int VM>
+__device__ inline void sched_slot() {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    if constexpr (DSR > 0) __builtin_amdgcn_sched_group_barrier(0x100, DSR, 0);
+    if constexpr (VA > 0) __builtin_amdgcn_sched_group_barrier(0x002, VA, 0);
+    if constexpr (DSW > 0) __builtin_amdgcn_sched_group_barrier(0x200, DSW, 0);
+    if constexpr (VM > 0) __builtin_amdgcn_sched_group_barrier(0x020, VM, 0);
+}
+// LDS reads spread over all NMF slots, VALU work and LDS writes over the
+// first half, the global loads right after the last LDS write.
+template <int NMF, int DSR, int VA, int DSW, int VM, int... I>
+__device__ inline void sched_pattern(std::integer_sequence<int, I...>) {
+    constexpr int H = NMF / 2;
+    (sched_slot<(I * ((DSR + NMF - 1) / NMF) < DSR ? (DSR + NMF - 1) / NMF : 0),
+                (I < H ? (VA + H - 1) / H : 0), (I < H ? (DSW + H - 1) / H : 0), (I == H - 1 ? VM : 0)>(),
+     ...);
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int SPLIT, bool BIN, int SCHED, typename TC>
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64)
+gemm_bf16p_kernel(GemmArgs g) {
+    static_assert(!BIN || SPLIT == 0, "bf16 storage has no split pieces");
+    constexpr int WAVES_M = BM / WM;
+    constexpr int NT = (BM / WM) * (BN / WN) * 64;
+    constexpr int TM = WM / 32, TN = WN / 32;
+    constexpr int KP = BK + 8;
+    constexpr int EV = BIN ? 8 : 4;           // elements per 16-byte global vector
+    constexpr int KQ = BK / EV;
+    constexpr int AV = BM * KQ / NT, BV = BN * KQ / NT;
+    constexpr int NS = SPLIT + 1;
+    static_assert(AV * NT == BM * KQ && BV * NT == BN * KQ, "tile not divisible by threads");
+    static_assert(BK % 16 == 0, "BK must be a multiple of 16");
+    constexpr int ASZ = NS * BM * KP, BSZ = NS * BN * KP;   // one stage
+    __shared__ __attribute__((aligned(16))) __bf16 As[2 * ASZ];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2 * BSZ];
+    typedef typename std::conditional<BIN, u16x8, f32x4>::type VT;
+    typedef typename std::conditional<BIN, __bf16, float>::type ET;
+
+    int tm, tn;
+    tile_coords(g, tm, tn);
+    const int b = blockIdx.y;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const ET *__restrict__ A = static_cast<const ET *>(g.A) + (long long)b * g.sAb;
+    const ET *__restrict__ B = static_cast<const ET *>(g.B) + (long long)b * g.sBb;
+    TC *__restrict__ C = static_cast<TC *>(g.C) + (long long)b * g.sCb;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    VT ra[AV], rb[BV];
+    // Buffer loads: a vector outside the operand gets an offset past the
+    // resource's extent, which the hardware returns as zeros, so no load sits
+    // behind a branch.  The host guarantees both extents below 2^31 bytes.
+    constexpr int ES = (int)sizeof(ET);
+    constexpr unsigned OOB = 0x80000000u;
+    const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)A, (short)0, (int)(((long long)(g.M - 1) * g.rsA + g.K) * ES), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)B, (short)0, (int)(((long long)(g.N - 1) * g.csB + g.K) * ES), 0x00020000);
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < AV; ++i) {
+            const int e = tid + i * NT;
+            const int m = m0 + e / KQ, k = k0 + EV * (e % KQ);
+            const unsigned off = (m < g.M && k < g.K) ? (unsigned)((m * (int)g.rsA + k) * ES) : OOB;
+            ra[i] = __builtin_bit_cast(VT, __builtin_amdgcn_raw_buffer_load_b128(rsA, off, 0, 0));
+        }
+#pragma unroll
+        for (int i = 0; i < BV; ++i) {
+            const int e = tid + i * NT;
+            const int n = n0 + e / KQ, k = k0 + EV * (e % KQ);
+            const unsigned off = (n < g.N && k < g.K) ? (unsigned)((n * (int)g.csB + k) * ES) : OOB;
+            rb[i] = __builtin_bit_cast(VT, __builtin_amdgcn_raw_buffer_load_b128(rsB, off, 0, 0));
+        }
+    };
+    auto store = [&](int stage) {
+#pragma unroll
+        for (int i = 0; i < AV; ++i) {
+            const int e = tid + i * NT;
+            const int off = stage * ASZ + (e / KQ) * KP + EV * (e % KQ);
+            if constexpr (BIN) *reinterpret_cast<u16x8 *>(&As[off]) = ra[i];
+            else put_pieces<SPLIT>(As, BM * KP, off, ra[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < BV; ++i) {
+            const int e = tid + i * NT;
+            const int off = stage * BSZ + (e / KQ) * KP + EV * (e % KQ);
+            if constexpr (BIN) *reinterpret_cast<u16x8 *>(&Bs[off]) = rb[i];
+            else put_pieces<SPLIT>(Bs, BN * KP, off, rb[i]);
+        }
+    };
+
+    const int nk = (g.K + BK - 1) / BK;
+    if (nk > 0) {
+        load(0);
+        store(0);
+        load(min(1, nk - 1) * BK);
+        __syncthreads();
+        int cur = 0;
+        for (int t = 0; t + 1 < nk; ++t) {
+            // the stage written here was last read before the previous barrier
+            store(cur ^ 1);
+            load(min(t + 2, nk - 1) * BK);
+            // SCHED 2: keep the conversion, LDS writes and global loads ahead
+            // of every MFMA of the slice (the loads then have a whole slice of
+            // MFMAs to land)
+            if constexpr (SCHED == 2) __builtin_amdgcn_sched_barrier(0);
+            bf16_slice<BM, BN, BK, KP, WM, WN, SPLIT>(As + cur * ASZ, Bs + cur * BSZ, acc, wm, wn, lane);
+            if constexpr (SCHED == 1) {
+                constexpr int MFP = SPLIT == 0 ? 1 : (SPLIT == 1 ? 3 : 6);   // MFMAs per block product
+                constexpr int NMF = TM * TN * (BK / 16) * MFP;
+                constexpr int DSR0 = (TM + TN) * NS;                        // LDS reads of one k step
+                constexpr int DSR = DSR0 * (BK / 16 - 1);
+                constexpr int DSW = (AV + BV) * NS;
+                constexpr int VM = AV + BV;
+                constexpr int VA = BIN ? 0 : (AV + BV) * 4 * (SPLIT == 0 ? 1 : (SPLIT == 1 ? 3 : 5));
+                __builtin_amdgcn_sched_group_barrier(0x100, DSR0, 0);
+                sched_pattern<NMF, DSR, VA, DSW, VM>(std::make_integer_sequence<int, NMF>{});
+            }
+            __syncthreads();
+            cur ^= 1;
+        }
+        bf16_slice<BM, BN, BK, KP, WM, WN, SPLIT>(As + cur * ASZ, Bs + cur * BSZ, acc, wm, wn, lane);
+    }
+
+    store_c<float, TC, TM, TN, 16>(
+        g, C, acc, [&](int i, int r) { return m0 + wm * WM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); },
+        [&](int j) { return n0 + wn * WN + j * 32 + (lane & 31); });
+}
+
+// ---------------------------------------------------------------------------
 // Host dispatch.
 // ---------------------------------------------------------------------------
 template <typename T, int BM, int BN, int BK, int WM, int WN, int VEC>
@@ -865,6 +1048,33 @@ static int launch_bf16in(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t 
     return check_launch();
 }
 
+// Pipelined kernel: fp32 storage needs the K-contiguous layout of
+// gemm_bf16k_kernel (other layouts run the generic fallback tile, same bits);
+// bf16 storage needs that of gemm_bf16in_kernel.
+template <int BM, int BN, int BK, int WM, int WN, int SPLIT, bool BIN, int SCHED, typename TC = float>
+static int launch_bf16p(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
+    // the kernel's buffer loads address each operand instance with 32-bit
+    // byte offsets and use 2^31 as the out-of-range offset
+    const long long es = BIN ? 2 : 4;
+    const bool fits = (((long long)(g.M - 1) * g.rsA + g.K) * es < (1LL << 31)) &&
+                      (((long long)(g.N - 1) * g.csB + g.K) * es < (1LL << 31));
+    if constexpr (BIN) {
+        if (!kvec8_ok(g, amc, bnc)) return TRG_UNSUPPORTED;
+        if (!fits) return launch_bf16in<128, 128, 32, 64, 64>(g, batch, amc, bnc, s);
+    } else {
+        if (!kvec_ok(g, amc, bnc) || !fits) {
+            if constexpr (SPLIT > 0) return launch_bf16<128, 128, 32, 64, 64, SPLIT, false, false, TC>(g, batch, amc, bnc, s);
+            else return launch_bf16<256, 128, 32, 64, 64, 0, false, false, TC>(g, batch, amc, bnc, s);
+        }
+    }
+    g.tiles_m = (int)cdiv(g.M, BM);
+    g.tiles_n = (int)cdiv(g.N, BN);
+    dim3 grid(g.tiles_m * g.tiles_n, batch);
+    dim3 block((BM / WM) * (BN / WN) * 64);
+    hipLaunchKernelGGL((gemm_bf16p_kernel<BM, BN, BK, WM, WN, SPLIT, BIN, SCHED, TC>), grid, block, 0, s, g);
+    return check_launch();
+}
+
 // Tile configuration table.  config -1 selects by shape (the default
 // heuristic).  Every fp32 configuration uses the same MFMA instruction and
 // walks K in the same order, so the choice never changes the result bits.
@@ -885,6 +1095,12 @@ static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmAr
             case 8: return launch_bf16in<256, 256, 64, 128, 64, true>(g, batch, amc, bnc, s);
             case 9: return launch_bf16in<256, 256, 128, 128, 64>(g, batch, amc, bnc, s);
             case 10: return launch_bf16in<128, 256, 64, 64, 64, true>(g, batch, amc, bnc, s);
+            case 11: return launch_bf16p<256, 256, 64, 128, 64, 0, true, 0>(g, batch, amc, bnc, s);
+            case 12: return launch_bf16p<256, 256, 64, 128, 64, 0, true, 1>(g, batch, amc, bnc, s);
+            case 13: return launch_bf16p<256, 256, 32, 128, 64, 0, true, 1>(g, batch, amc, bnc, s);
+            case 14: return launch_bf16p<256, 256, 64, 128, 64, 0, true, 2>(g, batch, amc, bnc, s);
+            case 15: return launch_bf16p<256, 256, 64, 128, 128, 0, true, 0>(g, batch, amc, bnc, s);
+            case 16: return launch_bf16p<256, 256, 64, 128, 128, 0, true, 2>(g, batch, amc, bnc, s);
             default: return TRG_BAD_ARG;
         }
     }
@@ -932,6 +1148,8 @@ static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmAr
             case 9: return launch_bf16<128, 256, 32, 64, 64, false>(g, batch, amc, bnc, s);
             case 10: return launch_bf16<256, 256, 32, 128, 64, false, true, true>(g, batch, amc, bnc, s);
             case 11: return launch_bf16<256, 128, 32, 64, 64, false, true, true>(g, batch, amc, bnc, s);
+            case 12: return launch_bf16p<256, 256, 32, 128, 64, 0, false, 0>(g, batch, amc, bnc, s);
+            case 13: return launch_bf16p<256, 256, 32, 128, 64, 0, false, 1>(g, batch, amc, bnc, s);
             default: return TRG_BAD_ARG;
         }
     }
@@ -946,6 +1164,12 @@ static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmAr
             case 5: return launch_bf16<256, 128, 32, 128, 64, true, true>(g, batch, amc, bnc, s);
             case 6: return launch_bf16<256, 128, 32, 64, 64, true, true, true>(g, batch, amc, bnc, s);
             case 7: return launch_bf16<128, 256, 32, 64, 64, true, true, true>(g, batch, amc, bnc, s);
+            case 8: return launch_bf16p<256, 256, 32, 128, 64, 1, false, 0>(g, batch, amc, bnc, s);
+            case 9: return launch_bf16p<256, 256, 32, 128, 64, 1, false, 1>(g, batch, amc, bnc, s);
+            case 10: return launch_bf16p<256, 128, 32, 64, 64, 1, false, 1>(g, batch, amc, bnc, s);
+            case 11: return launch_bf16p<256, 256, 32, 128, 64, 1, false, 2>(g, batch, amc, bnc, s);
+            case 12: return launch_bf16p<256, 256, 32, 128, 128, 1, false, 0>(g, batch, amc, bnc, s);
+            case 13: return launch_bf16p<256, 256, 32, 128, 128, 1, false, 2>(g, batch, amc, bnc, s);
             default: return TRG_BAD_ARG;
         }
     }
@@ -958,6 +1182,11 @@ static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmAr
             case 3: return launch_bf16<128, 256, 32, 64, 64, 2>(g, batch, amc, bnc, s);
             case 4: return launch_bf16<256, 256, 32, 128, 64, 2, true>(g, batch, amc, bnc, s);
             case 5: return launch_bf16<256, 128, 32, 128, 64, 2, true>(g, batch, amc, bnc, s);
+            case 6: return launch_bf16p<128, 128, 32, 64, 64, 2, false, 1>(g, batch, amc, bnc, s);
+            case 7: return launch_bf16p<256, 256, 16, 128, 64, 2, false, 0>(g, batch, amc, bnc, s);
+            case 8: return launch_bf16p<256, 256, 16, 128, 64, 2, false, 1>(g, batch, amc, bnc, s);
+            case 9: return launch_bf16p<256, 256, 16, 128, 64, 2, false, 2>(g, batch, amc, bnc, s);
+            case 10: return launch_bf16p<256, 256, 16, 128, 128, 2, false, 0>(g, batch, amc, bnc, s);
             default: return TRG_BAD_ARG;
         }
     }
@@ -1005,13 +1234,13 @@ extern "C" int trg_gemm(int dtype, int compute, int batch, int M, int N, int K, 
 }
 
 extern "C" int trg_gemm_config_count(int dtype, int c_dtype, int compute) {
-    if (dtype == TRG_BF16 && c_dtype == TRG_F32 && compute == TRG_BF16) return 11;
+    if (dtype == TRG_BF16 && c_dtype == TRG_F32 && compute == TRG_BF16) return 17;
     if (dtype == TRG_F32 && c_dtype == TRG_BF16 && compute == TRG_BF16) return 3;
     if (c_dtype != dtype) return 0;
     if (dtype == TRG_F32 && compute == TRG_F32) return 8;
-    if (dtype == TRG_F32 && compute == TRG_BF16) return 12;
-    if (dtype == TRG_F32 && compute == TRG_BF16X3) return 8;
-    if (dtype == TRG_F32 && compute == TRG_BF16X6) return 6;
+    if (dtype == TRG_F32 && compute == TRG_BF16) return 14;
+    if (dtype == TRG_F32 && compute == TRG_BF16X3) return 14;
+    if (dtype == TRG_F32 && compute == TRG_BF16X6) return 11;
     if (dtype == TRG_F64 && compute == TRG_F64) return 4;
     return 0;
 }
@@ -1022,16 +1251,23 @@ extern "C" const char *trg_gemm_config_name(int dtype, int c_dtype, int compute,
     static const char *bf[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "256x128x32 w128x64",
                                "128x256x32 w64x128", "128x128x64 w64x64", "256x128x64 w128x64",
                                "256x256x32 w128x64", "256x256x16 w128x64", "128x256x32 w64x64",
-                               "256x256x32 w128x64 db", "256x128x32 w64x64 db"};
+                               "256x256x32 w128x64 db", "256x128x32 w64x64 db", "256x256x32 w128x64 pipe",
+                               "256x256x32 w128x64 pipe sched"};
     static const char *bx3[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "128x256x32 w64x64",
                                 "256x256x32 w128x64", "256x128x32 w128x64", "256x128x32 w64x64 db",
-                                "128x256x32 w64x64 db"};
+                                "128x256x32 w64x64 db", "256x256x32 w128x64 pipe", "256x256x32 w128x64 pipe sched",
+                                "256x128x32 w64x64 pipe sched", "256x256x32 w128x64 pipe phased",
+                                "256x256x32 w128x128 pipe", "256x256x32 w128x128 pipe phased"};
     static const char *bin[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "256x256x32 w128x64",
                                 "256x256x64 w128x64", "256x256x32 w128x64 db", "256x128x64 w64x64 db",
                                 "128x256x32 w64x64", "256x256x64 w128x64 db", "256x256x128 w128x64",
-                                "128x256x64 w64x64 db"};
+                                "128x256x64 w64x64 db", "256x256x64 w128x64 pipe", "256x256x64 w128x64 pipe sched",
+                                "256x256x32 w128x64 pipe sched", "256x256x64 w128x64 pipe phased",
+                                "256x256x64 w128x128 pipe", "256x256x64 w128x128 pipe phased"};
     static const char *bx6[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "128x256x32 w64x64",
-                                "256x256x32 w128x64", "256x128x32 w128x64"};
+                                "256x256x32 w128x64", "256x128x32 w128x64", "128x128x32 w64x64 pipe sched",
+                                "256x256x16 w128x64 pipe", "256x256x16 w128x64 pipe sched",
+                                "256x256x16 w128x64 pipe phased", "256x256x16 w128x128 pipe"};
     static const char *f64[] = {"128x128x16 w64x64", "64x64x16 w32x32", "128x64x16 w64x32", "64x128x16 w32x64"};
     if (config < 0 || config >= trg_gemm_config_count(dtype, c_dtype, compute)) return "";
     if (dtype == TRG_BF16) return bin[config];
