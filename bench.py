@@ -82,6 +82,9 @@ def parse():
     ap.add_argument("--no-graphs", action="store_true", help="hotrg3d: eager steady steps instead of HIP graphs")
     ap.add_argument("--load-tune", default=None, help="GEMM tile table (JSON) to start from")
     ap.add_argument("--save-tune", default=None, help="write the GEMM tile table after the warm-up")
+    ap.add_argument("--gemm-sweep", action="store_true",
+                    help="time every tile configuration of the 2D chi=64 merge GEMM (bf16, split-bf16, three-piece) "
+                         "and check each against the first, bit for bit")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run one steady step with each merge precision plus a 1 GiB copy of known size "
                          "(for hardware-counter runs), then exit")
@@ -367,6 +370,53 @@ def bench_serve(args):
     return 0
 
 
+def gemm_sweep(args):
+    """Every tile configuration of the chi=64 merge GEMM of the 2D solve (M = 8 chi^2, N = K = chi^2,
+    batch 32, the solve's two-level output index), per compute type: average of 3 launches after one
+    warm-up launch, and bitwise equality with configuration 4 (the 256x256 K-contiguous tile)."""
+    import torch
+
+    from trg_backup_amd import _lib, ops
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    lib = _lib.load()
+    bt, M, N, K = 32, 32768, 4096, 4096
+    torch.manual_seed(0)
+    A = torch.randn(bt, M, K, device="cuda")
+    B = torch.randn(bt, N, K, device="cuda")           # B(k, n) at n*K + k
+    C, Cref = torch.empty(bt * M * N, device="cuda"), torch.empty(bt * M * N, device="cuda")
+    lay = (M * N, 32768, 1, 4096, 64, 64, 512)          # [k,i,l,c,d] destination of the merge
+    flops = 2.0 * M * N * K * bt
+    rows = []
+    for name, dt, cc in (("bf16 (bf16 operands)", _lib.BF16, _lib.BF16), ("split-bf16", _lib.F32, _lib.BF16X3),
+                         ("three-piece split", _lib.F32, _lib.BF16X6)):
+        a, b = (A.to(torch.bfloat16), B.to(torch.bfloat16)) if dt == _lib.BF16 else (A, B)
+        n = lib.trg_gemm_config_count(dt, _lib.F32, cc)
+        for cfg in [4] + [c for c in range(n) if c != 4]:
+            out = Cref if cfg == 4 else C
+
+            def go():
+                _lib.call("trg_gemm_ex", cfg, dt, _lib.F32, cc, bt, M, N, K, 1.0, a.data_ptr(), M * K, K, 1,
+                          b.data_ptr(), N * K, 1, K, 0.0, out.data_ptr(), *lay, ops.stream_ptr())
+            go()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record()
+            for _ in range(3):
+                go()
+            ev[1].record()
+            ev[1].synchronize()
+            ms = ev[0].elapsed_time(ev[1]) / 3
+            row = {"compute": name, "config": cfg,
+                   "tile": lib.trg_gemm_config_name(dt, _lib.F32, cc, cfg).decode(), "ms": ms,
+                   "tflops": flops / ms / 1e9, "bitwise_equal_to_config_4": cfg == 4 or bool(torch.equal(C, Cref))}
+            rows.append(row)
+            log(f"{name:22s} cfg {cfg:2d} {row['tile']:32s} {ms:8.2f} ms {row['tflops']:7.1f} TF/s "
+                f"equal={row['bitwise_equal_to_config_4']}")
+        del a, b
+        torch.cuda.empty_cache()
+    print(json.dumps({"gemm_sweep": rows}), flush=True)
+    return 0 if all(r["bitwise_equal_to_config_4"] for r in rows) else 1
+
+
 def traffic_probe(args):
     """One steady step of the bench batch with each merge precision (float32, bf16x3, bf16),
     then a plain 1 GiB float32 copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
@@ -423,6 +473,8 @@ def main():
     args.steps = args.steps or {"hotrg2d": 40, "hotrg3d": 6, "hotrg3d_qr": 6, "atrg3d": 30}.get(wl, 40)
     args.fp64_steps = (args.fp64_steps if args.fp64_steps is not None
                        else {"hotrg2d": 6, "hotrg3d": 4, "hotrg3d_qr": 10 ** 6, "atrg3d": 16}.get(wl, 0))
+    if args.gemm_sweep:
+        return gemm_sweep(args)
     if args.traffic_probe:
         return traffic_probe(args)
     if wl == "hotrg3d":

@@ -334,3 +334,76 @@ def ln_Z_over_V(T, Dcut, TOT_RGSTEPS):
     T = pure_tensor(T, Dcut, TOT_RGSTEPS)
     V = 2 ** sum(TOT_RGSTEPS.values())
     return float(np.real(np.sum(T.get_normalization_const()) + np.log(T.trace()) / V)), T
+
+
+# ---------------------------------------------------------------------------
+# one-point function (impurity) flow
+# ---------------------------------------------------------------------------
+def new_impuer_tensor_1point(T, T0, Dcut, direction):
+    """trg/ATRG_3d_new.py:669-734 (the list form; :614-667 is the one-tensor
+    case).  The squeezers come from the pure pair.  Each impurity forms its
+    own pair with a pure tensor: it is the lower tensor (C, D) when its
+    coordinate along `direction` is even and the upper one (A, B) when odd;
+    the pair gets its own leg swap and the pure pair's coarse graining."""
+    T = leg_transposition(T, "transpose", direction)
+    T0 = [leg_transposition(t, "transpose", direction) for t in T0]
+    A, D = T.U, T.VH
+    B = T.s[:, None, None, None] * T.VH
+    C = T.U * T.s[None, None, None, :]
+    UM, sM, VMH = leg_swapping(B, C, Dcut, ispure=True)
+    Px, PX, Py, PY = squeezer(A, UM, sM, VMH, D, Dcut)
+    for i, t in enumerate(T0):
+        if t.loc[direction] % 2 == 0:
+            Ai, Bi = T.U, T.s[:, None, None, None] * T.VH
+            Ci, Di = t.U * t.s[None, None, None, :], t.VH
+        else:
+            Ai, Bi = t.U, t.s[:, None, None, None] * t.VH
+            Ci, Di = T.U * T.s[None, None, None, :], T.VH
+        UMi, sMi, VMHi = leg_swapping(Bi, Ci, Dcut, ispure=False)
+        U, s, VH = coarse_graining(Ai, UMi, sMi, VMHi, Di, Px, PX, Py, PY, slicing=Dcut)
+        T0[i] = t.update(U, s, VH)
+    U, s, VH = coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY, slicing=Dcut)
+    T = T.update(U, s, VH)
+    T = leg_transposition(T, "restore", direction)
+    T0 = [leg_transposition(t, "restore", direction) for t in T0]
+    return T, T0
+
+
+def field_expected_value(T, T0, nsteps):
+    """trg/ATRG_3d_new.py:1043-1060: ln Z/V and P_i = exp(sum T0_i factors) Tr T0_i / Tr T
+    after `nsteps` RG steps (the factor of step n of T is ln c / 2^n)."""
+    trT = float(np.real(T.trace()))
+    V = 2 ** nsteps
+    lnz = float(np.real(np.sum(T.get_normalization_const()))) + math.log(trT) / V
+    P = [math.exp(float(np.real(np.sum(t.get_normalization_const())))) * float(np.real(t.trace())) / trT
+         for t in T0]
+    return lnz, P
+
+
+def one_point_function(T, T0, Dcut, TOT_RGSTEPS):
+    """trg/ATRG_3d_new.py:1079-1189: cycle T, X, Y; after each step
+    normalise T (factor ln c / 2^n) and each impurity (factor ln c0 - ln c),
+    then halve the impurity coordinates along the step's direction.
+    Returns (T, T0, lnZ/V, P)."""
+    rgstep = {"X": 0, "Y": 0, "T": 0}
+    key = lambda: f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
+    T, c = normalization(T)
+    T.normal_const[key()] = math.log(c)
+    for t in T0:
+        t, c0 = normalization(t)
+        t.normal_const[key()] = math.log(c0) - math.log(c)
+    total = sum(TOT_RGSTEPS.values())
+    for direction in itertools.cycle("TXY"):
+        if sum(rgstep.values()) >= total:
+            break
+        if rgstep[direction] < TOT_RGSTEPS[direction]:
+            rgstep[direction] += 1
+            T, T0 = new_impuer_tensor_1point(T, T0, Dcut, direction)
+            T, c = normalization(T)
+            T.normal_const[key()] = math.log(c) / 2 ** sum(rgstep.values())
+            for t in T0:
+                t, c0 = normalization(t)
+                t.normal_const[key()] = math.log(c0) - math.log(c)
+                t.loc[direction] //= 2
+    lnz, P = field_expected_value(T, T0, total)
+    return T, T0, lnz, P

@@ -307,3 +307,95 @@ def network_value(tensors, shape):
     for t, sb in zip(operands, subs):
         args += [t, sb]
     return float(np.einsum(*args, [], optimize="greedy"))
+
+
+# ---------------------------------------------------------------------------
+# two-point function (trg/HOTRG_3d_QR.py:428-518, 910-1033)
+# ---------------------------------------------------------------------------
+def sqr_distance(loc1, loc2):
+    """Squared distance of two location dicts (trg/HOTRG_3d_QR.py, as in trg/ATRG_3d_new.py:960-965)."""
+    return sum((loc1[k] - loc2[k]) ** 2 for k in loc1)
+
+
+def new_impuer_tensor_2points(T, T0, Tn, loc0, locn, Dcut, direction, squeezer_type="QR", degeneracy_eps=0.0):
+    """trg/HOTRG_3d_QR.py:428-475 on arrays: projectors from the pure tensor;
+    each impurity is merged with a pure partner, as the upper tensor of the
+    pair when its coordinate along `direction` is odd."""
+    T = leg_transposition(T, "transpose", direction)
+    T0 = leg_transposition(T0, "transpose", direction)
+    Tn = leg_transposition(Tn, "transpose", direction)
+    Px, PX, Py, PY, _ = squeezer(T, T, T, T, Dcut, squeezer_type, degeneracy_eps)
+    T0 = coarse_graining(T, T0, Px, PX, Py, PY) if loc0[direction] % 2 == 1 else coarse_graining(T0, T, Px, PX, Py, PY)
+    Tn = coarse_graining(T, Tn, Px, PX, Py, PY) if locn[direction] % 2 == 1 else coarse_graining(Tn, T, Px, PX, Py, PY)
+    T = coarse_graining(T, T, Px, PX, Py, PY)
+    return (leg_transposition(T, "restore", direction), leg_transposition(T0, "restore", direction),
+            leg_transposition(Tn, "restore", direction))
+
+
+def new_impuer_tensor_absorb(T, T0, Tn, loc0, locn, Dcut, direction, squeezer_type="QR", degeneracy_eps=0.0):
+    """trg/HOTRG_3d_QR.py:477-518 on arrays: the two impurities form one pair
+    (the even coordinate is the lower tensor) and become one impurity."""
+    T = leg_transposition(T, "transpose", direction)
+    T0 = leg_transposition(T0, "transpose", direction)
+    Tn = leg_transposition(Tn, "transpose", direction)
+    Px, PX, Py, PY, _ = squeezer(T, T, T, T, Dcut, squeezer_type, degeneracy_eps)
+    if loc0[direction] % 2 == 0 and locn[direction] % 2 == 1:
+        T0 = coarse_graining(T0, Tn, Px, PX, Py, PY)
+    elif loc0[direction] % 2 == 1 and locn[direction] % 2 == 0:
+        T0 = coarse_graining(Tn, T0, Px, PX, Py, PY)
+    T = coarse_graining(T, T, Px, PX, Py, PY)
+    return leg_transposition(T, "restore", direction), leg_transposition(T0, "restore", direction)
+
+
+def two_point_function(T, T0, Tn, Dcut, TOT_RGSTEPS, squeezer_type="QR", degeneracy_eps=0.0):
+    """trg/HOTRG_3d_QR.py:910-1033 on the dicts {'Tensor', 'loc', 'factor'}:
+    while both impurities exist, T0['factor'] holds ln c0 + ln cn - 2 ln c per
+    step; once their coordinates coincide after a step they are absorbed
+    into one impurity (factor ln c0 - ln c from then on; the one-point step
+    when Tn is gone).  Returns (T, T0, G) with G = exp(sum T0 factors)
+    Tr T0 / Tr T after the last step."""
+    rgstep = {"X": 0, "Y": 0, "Z": 0}
+    key = lambda: f"({rgstep['X']},{rgstep['Y']},{rgstep['Z']})"
+    T["Tensor"], c = normalization(T["Tensor"])
+    T0["Tensor"], c0 = normalization(T0["Tensor"])
+    Tn["Tensor"], c1 = normalization(Tn["Tensor"])
+    T["factor"][key()] = math.log(c)
+    T0["factor"][key()] = math.log(c0) + math.log(c1) - 2 * math.log(c)
+    have_n = True
+    for direction in cycle("ZXY"):
+        if sum(rgstep.values()) >= sum(TOT_RGSTEPS.values()):
+            break
+        if rgstep[direction] >= TOT_RGSTEPS[direction]:
+            continue
+        rgstep[direction] += 1
+        loc0, n = T0["loc"], sum(rgstep.values())
+        if have_n:
+            locn = Tn["loc"]
+            after0 = dict(loc0, **{direction: loc0[direction] // 2})
+            aftern = dict(locn, **{direction: locn[direction] // 2})
+            if sqr_distance(after0, aftern) > 0:
+                T["Tensor"], T0["Tensor"], Tn["Tensor"] = new_impuer_tensor_2points(
+                    T["Tensor"], T0["Tensor"], Tn["Tensor"], loc0, locn, Dcut, direction, squeezer_type,
+                    degeneracy_eps)
+                T["Tensor"], c = normalization(T["Tensor"])
+                T0["Tensor"], c0 = normalization(T0["Tensor"])
+                Tn["Tensor"], c1 = normalization(Tn["Tensor"])
+                T["factor"][key()] = math.log(c) / 2 ** n
+                T0["factor"][key()] = math.log(c0) + math.log(c1) - 2 * math.log(c)
+                loc0[direction] //= 2
+                locn[direction] //= 2
+                continue
+            T["Tensor"], T0["Tensor"] = new_impuer_tensor_absorb(
+                T["Tensor"], T0["Tensor"], Tn["Tensor"], loc0, locn, Dcut, direction, squeezer_type, degeneracy_eps)
+            have_n = False
+        else:
+            T["Tensor"], T0["Tensor"] = new_impuer_tensor_1point(T["Tensor"], T0["Tensor"], loc0, Dcut, direction,
+                                                                 squeezer_type, degeneracy_eps)
+        T["Tensor"], c = normalization(T["Tensor"])
+        T0["Tensor"], c0 = normalization(T0["Tensor"])
+        T["factor"][key()] = math.log(c) / 2 ** n
+        T0["factor"][key()] = math.log(c0) - math.log(c)
+        loc0[direction] //= 2
+    G = math.exp(sum(T0["factor"].values())) * float(np.einsum("xxyyzz", T0["Tensor"])) \
+        / float(np.einsum("xxyyzz", T["Tensor"]))
+    return T, T0, G

@@ -40,3 +40,25 @@ def test_bad_arguments_are_rejected_without_a_gpu():
     assert lib.trg_gemm(0, 0, 1, 4, 4, 4, 1.0, None, 0, 1, 4, None, 0, 4, 1, 0.0, None, 0, 4, 1, None) == 1
     assert lib.trg_syevj_f64(1, 300, None, 0, None, None, 10, None, None) == 2
     assert lib.trg_eigh_topk_f64(1, 10, 5, 20, 2, None, 0, None, None, None, 0, None, 0, None, None) == 2
+
+
+def test_library_shares_torchs_hip_runtime_whatever_the_import_order():
+    """Loading the library before anything imported torch must still leave a
+    single HIP runtime in the process (torch's): the library takes torch's
+    device pointers and stream handles.  Run in a fresh interpreter, since
+    this one has imported torch already."""
+    import subprocess
+    import sys
+    from trg_backup_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libtrg.so not built (run __graft_entry__.build())")
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from trg_backup_amd import _lib\n"
+            "assert 'torch' not in sys.modules\n"
+            "_lib.load()\n"
+            "import torch\n"
+            "print(_lib.hip_runtimes())\n") % ROOT
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    rts = eval(out.stdout.strip().splitlines()[-1])
+    assert len(rts) == 1 and "torch" in rts[0], rts

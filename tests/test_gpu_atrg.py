@@ -225,3 +225,79 @@ def test_truncate_known_matches_full_spectrum(gpu, case):
     chi, _ = linalg.truncate_known(_dev(full[:known], gpu), _dev(np.array([tail_sq]), gpu),
                                    _dev(np.array([total]), gpu), k, deg, 1e-10)
     assert chi == ref[0], (chi, ref)
+
+
+def _atrg_pair(T6, chi, loc=None):
+    from trg_backup_amd.tensor_class.tensor_class import ATRG_Tensor
+    t = models.to_atrg(T6, chi)
+    return ATRG_Tensor(t.U, t.s, t.VH, chi, 3, loc is not None, dict(loc or {}))
+
+
+def test_atrg_one_point_equals_network_contraction(gpu):
+    """Device one-point flow, untruncated (Dcut above every rank), on the
+    2x2x2 box: P = Z[impurity at one site] / Z of the eight-tensor network
+    for two random impurities carried together, at sites of both parities;
+    ln Z/V equals the enumeration."""
+    from oracle import HOTRG_3d_QR as hq
+    from trg_backup_amd.trg import ATRG_3d_new as at
+    rng = np.random.default_rng(21)
+    T6 = models.ising3d_tensor(0.22)
+    imps = [rng.standard_normal((2,) * 6) + 1.5 * T6, rng.standard_normal((2,) * 6) + 2.0 * T6]
+    sites = [(i, j, k) for i in range(2) for j in range(2) for k in range(2)]
+    Z = hq.network_value({s: T6 for s in sites}, (2, 2, 2))
+    at.set_rsvd_params(k_factor=3, p=0, q=2)
+    try:
+        for loc in [(0, 0, 0), (1, 0, 1), (1, 1, 1)]:
+            ref = [hq.network_value({s: (Ti if s == loc else T6) for s in sites}, (2, 2, 2)) / Z for Ti in imps]
+            T = _atrg_pair(T6, 64)
+            T0 = [_atrg_pair(Ti, 64, {"X": loc[0], "Y": loc[1], "T": loc[2]}) for Ti in imps]
+            T, T0 = at.one_point_function(T, T0, 64, {"X": 1, "Y": 1, "T": 1})
+            lnz, P = at.cal_field_expected_value(T, T0, {"X": 1, "Y": 1, "T": 1})
+            print(f"\n[atrg 1pt] loc={loc} P={P} ref={ref}", flush=True)
+            assert abs(lnz - exact.brute_lnZ_ising3d(0.22)) < 1e-12
+            for p, r in zip(P, ref):
+                assert abs(p - r) < 1e-10 * abs(r)
+            assert all(t.loc == {"X": 0, "Y": 0, "T": 0} for t in T0)
+    finally:
+        at.set_rsvd_params()
+
+
+def test_atrg_one_point_matches_oracle_truncated(gpu):
+    """Truncated flow (chi 8, two steps per direction) against the oracle with
+    the same algorithm and sketch (identity leg-swap basis, converged range
+    finder, DGENERACY_EPS 1e-8 on both sides, as in the pure-flow parity
+    test): ln Z/V and both impurity values agree to rounding."""
+    from trg_backup_amd import atrg
+    from trg_backup_amd.trg import ATRG_3d_new as at
+    rng = np.random.default_rng(4)
+    T6 = models.ising3d_tensor(0.22)
+    imps6 = [T6 + 0.3 * rng.standard_normal((2,) * 6), 2.0 * T6 + 0.5 * rng.standard_normal((2,) * 6)]
+    loc = {"X": 3, "Y": 2, "T": 1}
+    steps = {"X": 2, "Y": 2, "T": 2}
+    chi = 8
+    oat.set_leg_swap_basis("identity")
+    oat.set_rsvd_params(k_factor=2, p=0, q=20, seed=1)
+    at.set_rsvd_params(k_factor=2, p=0, q=20, seed=1)
+    old = (oat.degeneracy_eps, atrg.degeneracy_eps)
+    oat.degeneracy_eps = atrg.degeneracy_eps = 1e-8
+    try:
+        To = models.to_atrg(T6, chi)
+        T0o = []
+        for Ti in imps6:
+            t = models.to_atrg(Ti, chi)
+            t.loc = dict(loc)
+            T0o.append(t)
+        _, _, lnz_ref, P_ref = oat.one_point_function(To, T0o, chi, dict(steps))
+        T, T0 = at.one_point_function(_atrg_pair(T6, chi), [_atrg_pair(Ti, chi, loc) for Ti in imps6], chi,
+                                      dict(steps))
+        lnz, P = at.cal_field_expected_value(T, T0, steps)
+    finally:
+        oat.set_leg_swap_basis("svd")
+        oat.set_rsvd_params()
+        at.set_rsvd_params()
+        oat.degeneracy_eps, atrg.degeneracy_eps = old
+    print(f"\n[atrg 1pt chi={chi}] lnz {lnz!r} vs {lnz_ref!r}; P {P} vs {P_ref}", flush=True)
+    assert abs(lnz - lnz_ref) < 1e-11 * abs(lnz_ref)
+    for p, r in zip(P, P_ref):
+        assert abs(p - r) < 1e-9 * abs(r)
+    assert [t.loc for t in T0] == [t.loc for t in T0o]

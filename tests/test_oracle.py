@@ -332,3 +332,63 @@ def test_hotrg3d_qr_one_point_equals_network_contraction():
                                             {"Tensor": T0, "loc": {"X": loc[0], "Y": loc[1], "Z": loc[2]},
                                              "factor": {}}, 16, {"X": 1, "Y": 1, "Z": 1}, kind)
             assert abs(P - Zi / Z) < 1e-11 * abs(Zi / Z), (loc, kind, P, Zi / Z)
+
+
+def test_atrg_one_point_equals_network_contraction():
+    """The ATRG one-point flow (untruncated: Dcut above every rank) on the
+    2x2x2 box gives P = Z[impurity at one site] / Z of the eight-tensor
+    network, for a random impurity at every site (both parities of each
+    coordinate) and two impurities carried in one flow."""
+    import itertools
+    from oracle import HOTRG_3d_QR as hq
+    at.set_rsvd_params(k_factor=3, p=0, q=2)          # the sketch covers every rank here
+    try:
+        rng = np.random.default_rng(21)
+        T6 = models.ising3d_tensor(0.22)
+        imps = [rng.standard_normal((2,) * 6) + 1.5 * T6, rng.standard_normal((2,) * 6) + 2.0 * T6]
+        sites = list(itertools.product(range(2), repeat=3))
+        Z = hq.network_value({s: T6 for s in sites}, (2, 2, 2))
+        for loc in sites:
+            ref = [hq.network_value({s: (Ti if s == loc else T6) for s in sites}, (2, 2, 2)) / Z for Ti in imps]
+            T = models.to_atrg(T6, 64)
+            T0 = []
+            for Ti in imps:
+                t = models.to_atrg(Ti, 64)
+                t.loc = {"X": loc[0], "Y": loc[1], "T": loc[2]}
+                T0.append(t)
+            _, _, lnz, P = at.one_point_function(T, T0, 64, {"X": 1, "Y": 1, "T": 1})
+            assert abs(lnz - exact.brute_lnZ_ising3d(0.22)) < 1e-12
+            for p, r in zip(P, ref):
+                assert abs(p - r) < 1e-11 * abs(r), (loc, p, r)
+    finally:
+        at.set_rsvd_params()
+
+
+@pytest.mark.parametrize("shape,pairs", [
+    ((4, 1, 1), [((0, 0, 0), (1, 0, 0)), ((0, 0, 0), (3, 0, 0)), ((1, 0, 0), (2, 0, 0)), ((2, 0, 0), (0, 0, 0))]),
+    ((1, 4, 1), [((0, 1, 0), (0, 2, 0)), ((0, 3, 0), (0, 0, 0))]),
+    ((1, 1, 4), [((0, 0, 1), (0, 0, 2)), ((0, 0, 0), (0, 0, 3))]),
+    ((2, 2, 2), [((0, 0, 0), (1, 1, 1)), ((1, 0, 1), (0, 1, 1))]),
+])
+def test_hotrg3d_qr_two_point_equals_network_contraction(shape, pairs):
+    """trg/HOTRG_3d_QR.py:910-1033 untruncated: G = Z(T0 at r0, Tn at rn) / Z
+    equals the direct contraction of the box's network, with two different
+    random (non-symmetric) impurities, so the value depends on the
+    displacement's sign as well as its length: rings of 4 along each axis pin
+    which tensor of a pair sits at the even coordinate, and the 2x2x2 box the
+    absorb step after moves along several axes."""
+    import itertools
+    rng = np.random.default_rng(9)
+    T = models.ising3d_tensor(0.3)
+    T0 = T + 0.4 * rng.standard_normal(T.shape)
+    Tn = 1.5 * T + 0.4 * rng.standard_normal(T.shape)
+    sites = list(itertools.product(*[range(L) for L in shape]))
+    Z = hq.network_value({s: T for s in sites}, shape)
+    steps = {"X": shape[0].bit_length() - 1, "Y": shape[1].bit_length() - 1, "Z": shape[2].bit_length() - 1}
+    for r0, rn in pairs:
+        Zi = hq.network_value({s: (T0 if s == r0 else Tn if s == rn else T) for s in sites}, shape)
+        _, _, G = hq.two_point_function({"Tensor": T, "loc": {}, "factor": {}},
+                                        {"Tensor": T0, "loc": dict(zip("XYZ", r0)), "factor": {}},
+                                        {"Tensor": Tn, "loc": dict(zip("XYZ", rn)), "factor": {}},
+                                        16, dict(steps))
+        assert abs(G - Zi / Z) < 1e-11 * abs(Zi / Z), (shape, r0, rn, G, Zi / Z)

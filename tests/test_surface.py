@@ -26,7 +26,13 @@ SURFACE = {
                         "exec_rgstep_2point_func", "two_point_function"],
     "trg.ATRG_3d_new": ["leg_transposition", "rsvd_for_ATRG_leg_swapping", "leg_swapping", "squeezer",
                         "leg_slicing", "coarse_graining", "atrg_pure_tensor", "normalization",
-                        "exec_pure_tensor_renorm", "pure_tensor"],
+                        "exec_pure_tensor_renorm", "pure_tensor", "new_impuer_tensor_2points",
+                        "new_impuer_tensor_absorb", "new_impuer_tensor_1point",
+                        "new_impuer_tensor_1point_MultipleImpureTensors", "save_singularvalues", "cal_X",
+                        "find_max", "find_1stto3rdmax", "cal_scaling_dimensions", "cal_coorelation",
+                        "sqr_distance", "cal_field_expected_value", "exec_rgstep_1point_func",
+                        "one_point_function", "cal_field_expected_value1", "exec_rgstep_1point_func1",
+                        "one_point_function1", "exec_rgstep_2point_func", "two_point_function"],
     "utility.truncated_svd": ["truncate", "svd", "eigh"],
     "utility.randomized_svd": ["rsvd", "rsvd2", "__power_iteration__", "__QR_iteration__",
                                "rsvd_for_3dATRG_tensor_init", "rsvd_for_3dATRG_tensor_inte"],

@@ -64,15 +64,39 @@ class TrgError(RuntimeError):
 _lib = None
 
 
+def hip_runtimes():
+    """Paths of the HIP runtime libraries mapped into this process (Linux)."""
+    found = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                path = line.split()[-1]
+                if "libamdhip64" in os.path.basename(path):
+                    found.add(path)
+    except OSError:
+        pass
+    return sorted(found)
+
+
 def load():
-    """Load libtrg.so (raises if it has not been built)."""
+    """Load libtrg.so (raises if it has not been built).
+
+    The library takes torch's device pointers and stream handles, so it must
+    run on the HIP runtime torch loaded.  torch is therefore imported first:
+    loaded before it, libtrg.so would bind its own copy of the runtime
+    (/opt/rocm's), torch would then bring a second one, and the library's
+    first launch on a torch stream fails (hipErrorNoDevice)."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise TrgError(f"{LIB_PATH} is not built; run `python -m trg_backup_amd.build` "
                        "(or __graft_entry__.build()) first")
+    import torch  # noqa: F401  (binds the process to torch's HIP runtime before the library loads)
     lib = ctypes.CDLL(LIB_PATH)
+    rts = hip_runtimes()
+    if len(rts) > 1:
+        raise TrgError(f"two HIP runtimes are mapped ({', '.join(rts)}): libtrg.so must share torch's")
     lib.trg_version.restype = ctypes.c_char_p
     got = lib.trg_version().decode()
     if f" abi {ABI} " not in got:

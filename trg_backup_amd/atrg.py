@@ -260,23 +260,51 @@ def coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY, compute=None):
     return U, sK, VH
 
 
-def atrg_pure_tensor(U, s, VH, Dcut, direction, prec="f64"):
-    """:441-482 -> (U, s, VH) of the coarse tensor (legs restored)."""
-    dt = _dt(prec)
+def _pair_operands(U, s, VH, direction, dt):
+    """Transposed legs and the pair operands of one tensor: A = U, D = VH,
+    B = s_i VH_{itxy}, C = U_{TXYj} s_j (trg/ATRG_3d_new.py:446-450)."""
     U, VH = ops.cast(U, dt), ops.cast(VH, dt)
     U, VH = leg_transposition(U, VH, "transpose", direction)
     sd = ops.cast(s, dt)
-    A, D = U, VH
     B = ops.contiguous(VH)
     ops.scale_rows_(B, sd, sd.numel(), VH.numel() // VH.shape[0], 0)    # s_i VH_{itxy}
     C = ops.contiguous(U)
     ops.scale_rows_(C, sd, sd.numel(), 1, 0)                             # U_{TXYj} s_j
+    return U, VH, B, C
+
+
+def atrg_pure_tensor(U, s, VH, Dcut, direction, prec="f64"):
+    """:441-482 -> (U, s, VH) of the coarse tensor (legs restored)."""
+    (U2, s2, VH2), _ = atrg_1point_step(U, s, VH, [], Dcut, direction, prec)
+    return U2, s2, VH2
+
+
+def atrg_1point_step(U, s, VH, imps, Dcut, direction, prec="f64"):
+    """One step of the pure tensor and of each impurity
+    (trg/ATRG_3d_new.py:669-734; with no impurity it is the pure step
+    :441-482).  `imps` holds (U0, s0, VH0, odd): odd is True when the
+    impurity's coordinate along `direction` is odd, and the impurity is
+    then the upper tensor (A, B) of its pair, else the lower one (C, D).
+    The squeezers come from the pure pair; each impurity pair gets its own
+    leg swap and the pure pair's coarse graining.  Returns
+    ((U, s, VH), [(U0, s0, VH0), ...]) with the legs restored."""
+    dt = _dt(prec)
+    A, D, B, C = _pair_operands(U, s, VH, direction, dt)
     UM, sM, VMH = leg_swapping(B, C, Dcut, True, prec)
     UM, VMH = ops.cast(UM, dt), ops.cast(VMH, dt)
     Px, PX, Py, PY = squeezer(A, UM, sM, VMH, D, Dcut)
+    out = []
+    for U0, s0, VH0, odd in imps:
+        A0, D0, B0, C0 = _pair_operands(U0, s0, VH0, direction, dt)
+        Ai, Bi, Ci, Di = (A0, B0, C, D) if odd else (A, B, C0, D0)
+        UMi, sMi, VMHi = leg_swapping(Bi, Ci, Dcut, False, prec)
+        UMi, VMHi = ops.cast(UMi, dt), ops.cast(VMHi, dt)
+        Ui, si, VHi = coarse_graining(Ai, UMi, sMi, VMHi, Di, Px, PX, Py, PY)
+        Ui, VHi = leg_transposition(Ui, VHi, "restore", direction)
+        out.append((Ui, si, VHi))
     U2, s2, VH2 = coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY)
     U2, VH2 = leg_transposition(U2, VH2, "restore", direction)
-    return U2, s2, VH2
+    return (U2, s2, VH2), out
 
 
 def trace(U, s, VH):

@@ -1076,13 +1076,14 @@ static int launch_bf16p(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s
 }
 
 // Tile configuration table.  config -1 selects by shape (the default
-// heuristic).  Every fp32 configuration uses the same MFMA instruction and
+// heuristic: the pipelined 256x256 tiles once a launch holds at least 512 of
+// them, otherwise the generic tiles).  Every fp32 configuration uses the same MFMA instruction and
 // walks K in the same order, so the choice never changes the result bits.
 static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmArgs g, int batch, bool amc,
                          bool bnc, hipStream_t s) {
     const long long work = (long long)g.M * g.N;
     if (dtype == TRG_BF16 && c_dtype == TRG_F32 && compute == TRG_BF16) {
-        if (config < 0) config = work >= 256LL * 256 * 64 ? 3 : 1;
+        if (config < 0) config = work * batch >= 256LL * 256 * 512 ? 14 : (work >= 256LL * 256 * 64 ? 3 : 1);
         switch (config) {
             case 0: return launch_bf16in<128, 128, 32, 64, 64>(g, batch, amc, bnc, s);
             case 1: return launch_bf16in<64, 64, 32, 32, 32>(g, batch, amc, bnc, s);
@@ -1154,7 +1155,7 @@ static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmAr
         }
     }
     if (dtype == TRG_F32 && compute == TRG_BF16X3) {
-        if (config < 0) config = work >= 256LL * 128 * 128 ? 0 : 1;
+        if (config < 0) config = work * batch >= 256LL * 256 * 512 ? 11 : (work >= 256LL * 128 * 128 ? 0 : 1);
         switch (config) {
             case 0: return launch_bf16<128, 128, 32, 64, 64, true>(g, batch, amc, bnc, s);
             case 1: return launch_bf16<64, 64, 32, 32, 32, true>(g, batch, amc, bnc, s);
@@ -1174,7 +1175,7 @@ static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmAr
         }
     }
     if (dtype == TRG_F32 && compute == TRG_BF16X6) {
-        if (config < 0) config = work >= 256LL * 128 * 128 ? 0 : 1;
+        if (config < 0) config = work * batch >= 256LL * 256 * 512 ? 9 : (work >= 256LL * 128 * 128 ? 0 : 1);
         switch (config) {
             case 0: return launch_bf16<128, 128, 32, 64, 64, 2>(g, batch, amc, bnc, s);
             case 1: return launch_bf16<64, 64, 32, 32, 32, 2>(g, batch, amc, bnc, s);

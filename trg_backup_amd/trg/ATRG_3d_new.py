@@ -4,10 +4,12 @@ Same names as trg/ATRG_3d_new.py:18-482 and 954-1039.  ``pure_tensor``
 takes an ATRG_Tensor (U, s, VH as NumPy arrays or GPU tensors) and returns it
 coarse grained, with normal_const filled as in :967-972.  Environment:
 DGENERACY_EPS (default 0.0, the reference requires it, :15); the per-step
-files under OUTPUT_DIR (:197-212, :736-766) are not written.  The impurity
-and scaling-dimension flows (:485-951, :1043-1489) are outside this module
-(see the package documentation).  ``set_precision("f32")`` runs the
-contractions in float32 on MFMA; decompositions stay float64.
+files under OUTPUT_DIR (:197-212, :736-766) are not written.  The
+one-point impurity flow (:614-734, :1043-1189) runs on the device with the
+pure tensor; the two-point and scaling-dimension flows (:485-613,
+:768-951, :1191-1489) are not on the device and raise NotImplementedError.
+``set_precision("f32")`` runs the contractions in float32 on MFMA;
+decompositions stay float64.
 """
 import itertools
 import math
@@ -130,11 +132,19 @@ def exec_pure_tensor_renorm(T, Dcut, direction, rgstep):
     return T
 
 
+def _reset_steps():
+    """Zero the module's step counters and return them (the flows below
+    advance them in place, as the reference's globals, :975-982)."""
+    global rgstep
+    rgstep.update({"X": 0, "Y": 0, "T": 0})
+    return rgstep
+
+
 def pure_tensor(T, Dcut, TOT_RGSTEPS):
     """:974-1039: cycle T, X, Y until the totals are reached.  The tensor stays
     on the GPU between steps and is returned in the caller's array kind."""
     U, s, VH, kind = _dev3(T)
-    rgstep = {"X": 0, "Y": 0, "T": 0}
+    rgstep = _reset_steps()
     s, c = atrg.normalize(U, s, VH)
     T.normal_const[f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"] = math.log(abs(float(c[0].item())))
     total = sum(TOT_RGSTEPS.values())
@@ -158,3 +168,173 @@ def ln_Z_over_V(T, Dcut, TOT_RGSTEPS):
     tr = abs(float(atrg.trace(U, s, VH)[0].item()))
     V = 2 ** sum(TOT_RGSTEPS.values())
     return sum(T.normal_const.values()) + math.log(tr) / V, T
+
+
+# ---------------------------------------------------------------------------
+# one-point function (impurity) flow, :614-734 and :1043-1189
+# ---------------------------------------------------------------------------
+def _imp_args(T0, direction):
+    out = []
+    for t in T0:
+        U0, s0, VH0, _ = _dev3(t)
+        out.append((U0, s0, VH0, t.loc[direction] % 2 == 1))
+    return out
+
+
+def new_impuer_tensor_1point_MultipleImpureTensors(T, T0, Dcut, direction):
+    """:669-734: one step of the pure tensor T and of every impurity in the
+    list T0 (each ATRG_Tensor with ``loc``), all updated in place."""
+    for i, t in enumerate(T0):
+        assert hasattr(t, "U") and hasattr(t, "loc"), f"T0[{i}] must be ATRG_Tensor type"
+    U, s, VH, kind = _dev3(T)
+    (U, s, VH), outs = atrg.atrg_1point_step(U, s, VH, _imp_args(T0, direction), Dcut, direction,
+                                             PRECISION["contract"])
+    T.update(from_device(U, kind), from_device(s, kind), from_device(VH, kind))
+    for t, (U0, s0, VH0) in zip(T0, outs):
+        t.update(from_device(U0, kind), from_device(s0, kind), from_device(VH0, kind))
+    return T, T0
+
+
+def new_impuer_tensor_1point(T, T0, Dcut, direction):
+    """:614-667: the one-impurity form of the step above."""
+    T, (T0,) = new_impuer_tensor_1point_MultipleImpureTensors(T, [T0], Dcut, direction)
+    return T, T0
+
+
+def exec_rgstep_1point_func(T, T0, Dcut, direction, rgstep):
+    """:1079-1101: step, normalise T (factor ln c / 2^n) and every impurity
+    (factor ln c0 - ln c), halve the impurity coordinates along `direction`."""
+    key = f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
+    T, T0 = new_impuer_tensor_1point_MultipleImpureTensors(T, T0, Dcut, direction)
+    T, c = normalization(T)
+    T.normal_const[key] = math.log(c) / 2 ** (sum(rgstep.values()))
+    for t in T0:
+        t, c0 = normalization(t)
+        t.normal_const[key] = math.log(c0) - math.log(c)
+        t.loc[direction] //= 2
+    return T, T0
+
+
+def cal_field_expected_value(T, T0, rgstep, save=False):
+    """:1043-1076: ln Z/V and P_i = exp(sum of T0[i]'s factors) Tr T0[i] / Tr T.
+    Returns (lnZ/V, [P_i]); with ``save`` the line "nsteps lnZ/V P_0 P_1" is
+    written to OUTPUT_DIR/field_expected_value_rgsteps_dependence.dat like the
+    reference (new file at step 0, appended after)."""
+    U, s, VH, _ = _dev3(T)
+    trT = float(atrg.trace(U, s, VH)[0].item())
+    n = sum(rgstep.values())
+    lnz = float(sum(T.normal_const.values())) + math.log(trT) / 2 ** n
+    P = []
+    for t in T0:
+        U0, s0, VH0, _ = _dev3(t)
+        P.append(math.exp(float(sum(t.normal_const.values()))) * float(atrg.trace(U0, s0, VH0)[0].item()) / trT)
+    if save:
+        out_dir = os.environ.get("OUTPUT_DIR")
+        if out_dir:
+            with open(os.path.join(out_dir, "field_expected_value_rgsteps_dependence.dat"),
+                      "w" if n == 0 else "a") as f:
+                f.write(f"{n} {lnz:.12e} " + " ".join(f"{p:.12e}" for p in P[:2]) + "\n")
+    return lnz, P
+
+
+def one_point_function(T, T0, Dcut, TOT_RGSTEPS):
+    """:1103-1189: cycle T, X, Y until the totals are reached, carrying the
+    impurities T0 (a list of ATRG_Tensor with ``loc``) with the pure tensor.
+    The tensors stay on the GPU between steps and are returned in the
+    caller's array kind; the per-step singular-value files (:736-748) are not
+    written."""
+    for i, t in enumerate(T0):
+        assert hasattr(t, "U") and hasattr(t, "loc"), f"T0[{i}] must be ATRG_Tensor type"
+    U, s, VH, kind = _dev3(T)
+    imps = [list(_dev3(t)[:3]) for t in T0]
+    rgstep = _reset_steps()
+
+    def norm_all(n_total):
+        nonlocal s
+        key = f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
+        s, c = atrg.normalize(U, s, VH)
+        lc = math.log(abs(float(c[0].item())))
+        T.normal_const[key] = lc / 2 ** n_total
+        for t, imp in zip(T0, imps):
+            imp[1], c0 = atrg.normalize(*imp)
+            t.normal_const[key] = math.log(abs(float(c0[0].item()))) - lc
+
+    norm_all(0)
+    total = sum(TOT_RGSTEPS.values())
+    for direction in itertools.cycle("TXY"):
+        if sum(rgstep.values()) >= total:
+            break
+        if rgstep[direction] < TOT_RGSTEPS[direction]:
+            rgstep[direction] += 1
+            args = [(u, sv, vh, t.loc[direction] % 2 == 1) for t, (u, sv, vh) in zip(T0, imps)]
+            (U, s, VH), outs = atrg.atrg_1point_step(U, s, VH, args, Dcut, direction,
+                                                     _step_precision(sum(rgstep.values())))
+            imps = [list(o) for o in outs]
+            norm_all(sum(rgstep.values()))
+            for t in T0:
+                t.loc[direction] //= 2
+    T.update(from_device(U, kind), from_device(s, kind), from_device(VH, kind))
+    for t, (U0, s0, VH0) in zip(T0, imps):
+        t.update(from_device(U0, kind), from_device(s0, kind), from_device(VH0, kind))
+    return T, T0
+
+
+# ---------------------------------------------------------------------------
+# small helpers and output files (:736-766, :960-965)
+# ---------------------------------------------------------------------------
+# step counters of the running flow, like the reference's module globals
+# (:975-982); save_singularvalues and cal_X name their output by them
+rgstep = {"X": 0, "Y": 0, "T": 0}
+
+
+def sqr_distance(loc1, loc2):
+    """:960-965: squared distance of two location dicts."""
+    return sum((a - b) * (a - b) for a, b in zip(loc1.values(), loc2.values()))
+
+
+def save_singularvalues(T, k):
+    """:736-748: s / max(s), zero-padded to max(k, len(s)), to
+    OUTPUT_DIR/tensor_sv/tensor_sv_lx{X}_ly{Y}_lt{T}.dat (no-op without OUTPUT_DIR)."""
+    out_dir = os.environ.get("OUTPUT_DIR")
+    if not out_dir:
+        return
+    s, _ = to_device(T.s)
+    s = s.double().cpu().numpy() if hasattr(s, "cpu") else s
+    d = os.path.join(out_dir, "tensor_sv")
+    os.makedirs(d, exist_ok=True)
+    smax = float(max(s))
+    vals = list(s / smax) + [0.0] * (max(k, len(s)) - len(s))
+    with open(os.path.join(d, f"tensor_sv_lx{rgstep['X']}_ly{rgstep['Y']}_lt{rgstep['T']}.dat"), "w") as f:
+        f.write(f"#s1={smax:.12e}\n")
+        f.writelines(f"{v:.12e}\n" for v in vals)
+
+
+def cal_X(T, save=False):
+    """:750-766: the X ratios of tensor_class.ATRG_Tensor.cal_X; with ``save``
+    one line "Xt Xx Xy" goes to OUTPUT_DIR/X.dat (new file at step 0)."""
+    Xt, Xx, Xy = T.cal_X()
+    out_dir = os.environ.get("OUTPUT_DIR")
+    if save and out_dir:
+        with open(os.path.join(out_dir, "X.dat"), "w" if sum(rgstep.values()) == 0 else "a") as f:
+            f.write(f"{float(Xt):.12e} {float(Xx):.12e} {float(Xy):.12e}\n")
+    return Xt, Xx, Xy
+
+
+def _not_on_device(name, lines):
+    def f(*args, **kwargs):
+        raise NotImplementedError(f"ATRG_3d_new.{name} (trg/ATRG_3d_new.py:{lines}) is not on the device")
+    f.__name__ = name
+    return f
+
+
+new_impuer_tensor_2points = _not_on_device("new_impuer_tensor_2points", "485-555")
+new_impuer_tensor_absorb = _not_on_device("new_impuer_tensor_absorb", "557-612")
+find_max = _not_on_device("find_max", "768-816")
+find_1stto3rdmax = _not_on_device("find_1stto3rdmax", "818-852")
+cal_scaling_dimensions = _not_on_device("cal_scaling_dimensions", "854-877")
+cal_coorelation = _not_on_device("cal_coorelation", "879-951")
+cal_field_expected_value1 = _not_on_device("cal_field_expected_value1", "1191-1222")
+exec_rgstep_1point_func1 = _not_on_device("exec_rgstep_1point_func1", "1224-1244")
+one_point_function1 = _not_on_device("one_point_function1", "1246-1331")
+exec_rgstep_2point_func = _not_on_device("exec_rgstep_2point_func", "1333-1417")
+two_point_function = _not_on_device("two_point_function", "1419-1489")

@@ -9,8 +9,8 @@ which only splits the reference's contraction loop and does not change the
 result.  The diagnostics written on every step (cal_X, cal_coorelation,
 cal_scaling_dimensions, prints and timings) are not emitted.  The squeezers
 take the pure-tensor form (T0 = T1 = T2 = T3); of the impurity flows the
-one-point function (:520-557, :798-908) is on the device, the two-point
-flows (:428-518, :910-1033) and the diagnostics are not.
+one-point (:520-557, :798-908) and two-point (:428-518, :910-1033) flows
+are on the device; the diagnostics are not.
 """
 import math
 import os
@@ -258,11 +258,109 @@ def _not_on_device(name, lines):
     return f
 
 
-new_impuer_tensor_2points = _not_on_device("new_impuer_tensor_2points", "428-475")
-new_impuer_tensor_absorb = _not_on_device("new_impuer_tensor_absorb", "477-518")
 cal_X = _not_on_device("cal_X", "559-583")
 find_1stto3rdmax = _not_on_device("find_1stto3rdmax", "585-611")
 cal_scaling_dimensions = _not_on_device("cal_scaling_dimensions", "613-637")
 cal_coorelation = _not_on_device("cal_coorelation", "639-713")
-exec_rgstep_2point_func = _not_on_device("exec_rgstep_2point_func", "910-995")
-two_point_function = _not_on_device("two_point_function", "997-1033")
+
+
+# ---------------------------------------------------------------------------
+# two-point function (trg/HOTRG_3d_QR.py:428-518, 910-1033)
+# ---------------------------------------------------------------------------
+def sqr_distance(loc1, loc2):
+    """Squared distance of two location dicts."""
+    return sum((loc1[k] - loc2[k]) ** 2 for k in loc1)
+
+
+def _squeeze_pure(Tb, Dcut, direction):
+    (_, U1a, U1b, U2a, U2b), = hotrg_qr.squeeze(Tb, Dcut, select_squeezer, degeneracy_eps, _prec_of(Tb))
+    return U1a, U1b, U2a, U2b
+
+
+def _merge(lo, up, P, prec):
+    return hotrg.merge3d(lo, up, *P, compute=prec.merge_compute())
+
+
+def new_impuer_tensor_2points(T, T0, Tn, Dcut, direction):
+    """trg/HOTRG_3d_QR.py:428-475 on the dicts: projectors from the pure
+    tensor; each impurity is merged with a pure partner, as the upper tensor
+    of its pair when its coordinate along `direction` is odd."""
+    (Td, kind), (T0d, _), (Tnd, _) = to_device(T["Tensor"]), to_device(T0["Tensor"]), to_device(Tn["Tensor"])
+    prec = _prec_of(Td)
+    Tb, T0b, Tnb = (hotrg.rearrange3d(ops.contiguous(x[None]), direction) for x in (Td, T0d, Tnd))
+    P = _squeeze_pure(Tb, Dcut, direction)
+    pair = lambda Xb, odd: _merge(Tb, Xb, P, prec) if odd else _merge(Xb, Tb, P, prec)
+    T0n = pair(T0b, T0["loc"][direction] % 2 == 1)
+    Tnn = pair(Tnb, Tn["loc"][direction] % 2 == 1)
+    Tn_pure = _merge(Tb, Tb, P, prec)
+    T["Tensor"] = from_device(hotrg.restore3d(Tn_pure, direction)[0], kind)
+    T0["Tensor"] = from_device(hotrg.restore3d(T0n, direction)[0], kind)
+    Tn["Tensor"] = from_device(hotrg.restore3d(Tnn, direction)[0], kind)
+    return T, T0, Tn
+
+
+def new_impuer_tensor_absorb(T, T0, Tn, Dcut, direction):
+    """trg/HOTRG_3d_QR.py:477-518 on the dicts: the two impurities form one
+    pair (the one at the even coordinate is the lower tensor) and T0 becomes
+    their merged impurity; returns (T, T0)."""
+    (Td, kind), (T0d, _), (Tnd, _) = to_device(T["Tensor"]), to_device(T0["Tensor"]), to_device(Tn["Tensor"])
+    prec = _prec_of(Td)
+    Tb, T0b, Tnb = (hotrg.rearrange3d(ops.contiguous(x[None]), direction) for x in (Td, T0d, Tnd))
+    P = _squeeze_pure(Tb, Dcut, direction)
+    p0, pn = T0["loc"][direction] % 2, Tn["loc"][direction] % 2
+    if p0 == 0 and pn == 1:
+        T0["Tensor"] = from_device(hotrg.restore3d(_merge(T0b, Tnb, P, prec), direction)[0], kind)
+    elif p0 == 1 and pn == 0:
+        T0["Tensor"] = from_device(hotrg.restore3d(_merge(Tnb, T0b, P, prec), direction)[0], kind)
+    T["Tensor"] = from_device(hotrg.restore3d(_merge(Tb, Tb, P, prec), direction)[0], kind)
+    return T, T0
+
+
+def _norm_dict(X):
+    Xd, kind = to_device(X["Tensor"])
+    Xn, c = _trace_norm(Xd)
+    X["Tensor"] = from_device(Xn, kind)
+    return c
+
+
+def exec_rgstep_2point_func(T, T0, Tn, Dcut, direction, rgstep):
+    """trg/HOTRG_3d_QR.py:910-995: while both impurities exist and their
+    coordinates stay apart after the step, the two-point step with factor
+    ln c0 + ln cn - 2 ln c for T0; when they meet, the absorb step (factor
+    ln c0 - ln c, Tn becomes {}); with Tn already gone, the one-point step.
+    Returns (T, T0, Tn)."""
+    key = f"({rgstep['X']},{rgstep['Y']},{rgstep['Z']})"
+    n = sum(rgstep.values())
+    if len(Tn) > 0:
+        after0 = dict(T0["loc"], **{direction: T0["loc"][direction] // 2})
+        aftern = dict(Tn["loc"], **{direction: Tn["loc"][direction] // 2})
+        if sqr_distance(after0, aftern) > 0:
+            T, T0, Tn = new_impuer_tensor_2points(T, T0, Tn, Dcut, direction)
+            c, c0, c1 = _norm_dict(T), _norm_dict(T0), _norm_dict(Tn)
+            T["factor"][key] = math.log(c) / 2 ** n
+            T0["factor"][key] = math.log(c0) + math.log(c1) - 2 * math.log(c)
+            T0["loc"][direction] //= 2
+            Tn["loc"][direction] //= 2
+            return T, T0, Tn
+        T, T0 = new_impuer_tensor_absorb(T, T0, Tn, Dcut, direction)
+    else:
+        T, T0 = new_impuer_tensor_1point(T, T0, Dcut, direction)
+    c, c0 = _norm_dict(T), _norm_dict(T0)
+    T["factor"][key] = math.log(c) / 2 ** n
+    T0["factor"][key] = math.log(c0) - math.log(c)
+    T0["loc"][direction] //= 2
+    return T, T0, {}
+
+
+def two_point_function(T, T0, Tn, Dcut, TOT_RGSTEPS):
+    """trg/HOTRG_3d_QR.py:997-1033: normalise the three tensors (T0's factor
+    ln c0 + ln cn - 2 ln c), then the "ZXY" cycle of exec_rgstep_2point_func.
+    Returns (T, T0); the correlator is cal_field_expected_value(T, [T0], ...)[1][0]."""
+    rgstep = {"X": 0, "Y": 0, "Z": 0}
+    c, c0, c1 = _norm_dict(T), _norm_dict(T0), _norm_dict(Tn)
+    T["factor"]["(0,0,0)"] = math.log(c)
+    T0["factor"]["(0,0,0)"] = math.log(c0) + math.log(c1) - 2 * math.log(c)
+    for direction in hotrg_qr.schedule(TOT_RGSTEPS):
+        rgstep[direction] += 1
+        T, T0, Tn = exec_rgstep_2point_func(T, T0, Tn, Dcut, direction, rgstep)
+    return T, T0
