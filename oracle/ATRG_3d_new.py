@@ -407,3 +407,106 @@ def one_point_function(T, T0, Dcut, TOT_RGSTEPS):
                 t.loc[direction] //= 2
     lnz, P = field_expected_value(T, T0, total)
     return T, T0, lnz, P
+
+
+# ---------------------------------------------------------------------------
+# two-point function (impurity) flow
+# ---------------------------------------------------------------------------
+def _pair_step(T, pairs, Dcut, direction):
+    """One step of the pure tensor T and of the impurity pairs: each pair is
+    (upper, lower) with None standing for the pure tensor; the upper tensor
+    gives A = U and B = s VH, the lower one C = U s and D = VH; the squeezers
+    come from the pure pair.  Returns (T, [new impurity per pair]), legs
+    restored (tensors are transposed in place, as the reference does)."""
+    T = leg_transposition(T, "transpose", direction)
+    for up, lo in pairs:
+        for t in (up, lo):
+            if t is not None:
+                leg_transposition(t, "transpose", direction)
+    A, D = T.U, T.VH
+    B = T.s[:, None, None, None] * T.VH
+    C = T.U * T.s[None, None, None, :]
+    UM, sM, VMH = leg_swapping(B, C, Dcut, ispure=True)
+    Px, PX, Py, PY = squeezer(A, UM, sM, VMH, D, Dcut)
+    outs = []
+    for up, lo in pairs:
+        u = T if up is None else up
+        l = T if lo is None else lo
+        Ai, Bi = u.U, u.s[:, None, None, None] * u.VH
+        Ci, Di = l.U * l.s[None, None, None, :], l.VH
+        UMi, sMi, VMHi = leg_swapping(Bi, Ci, Dcut, ispure=False)
+        outs.append(coarse_graining(Ai, UMi, sMi, VMHi, Di, Px, PX, Py, PY, slicing=Dcut))
+    U, s, VH = coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY, slicing=Dcut)
+    T = leg_transposition(T.update(U, s, VH), "restore", direction)
+    return T, outs
+
+
+def new_impuer_tensor_2points(T, T0, Tn, Dcut, direction):
+    """trg/ATRG_3d_new.py:485-555 as the one-point step applies it: each
+    impurity pairs with a pure tensor, as the upper tensor when its
+    coordinate along `direction` is odd, the lower one when even.  (The
+    reference's even branch, :519-521, contracts U with s and s with VH, and
+    :545 passes T as an extra first argument of coarse_graining; both fail
+    on the shapes.  This is the one-point step's pairing.)"""
+    pairs = [((t, None) if t.loc[direction] % 2 == 1 else (None, t)) for t in (T0, Tn)]
+    T, outs = _pair_step(T, pairs, Dcut, direction)
+    T0 = leg_transposition(T0.update(*outs[0]), "restore", direction)
+    Tn = leg_transposition(Tn.update(*outs[1]), "restore", direction)
+    return T, T0, Tn
+
+
+def new_impuer_tensor_absorb(T, T0, Tn, Dcut, direction):
+    """trg/ATRG_3d_new.py:557-612: the impurities form one pair (the one at the
+    odd coordinate is the upper tensor) and T0 becomes their merged impurity."""
+    up, lo = (Tn, T0) if T0.loc[direction] % 2 == 0 else (T0, Tn)
+    T, outs = _pair_step(T, [(up, lo)], Dcut, direction)
+    leg_transposition(Tn, "restore", direction)
+    T0 = leg_transposition(T0.update(*outs[0]), "restore", direction)
+    return T, T0
+
+
+def two_point_function(T, T0, Tn, Dcut, TOT_RGSTEPS):
+    """trg/ATRG_3d_new.py:1333-1489: cycle T, X, Y.  While the impurities are
+    apart after a step, the two-point step with T0's factor ln c0 + ln cn -
+    2 ln c; when they meet, the absorb step (factor ln c0 - ln c); afterwards
+    the one-point step.  Returns (T, T0, lnZ/V, G) with
+    G = exp(sum T0 factors) Tr T0 / Tr T."""
+    rgstep = {"X": 0, "Y": 0, "T": 0}
+    key = lambda: f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
+    T, c = normalization(T)
+    T0, c0 = normalization(T0)
+    Tn, c1 = normalization(Tn)
+    T.normal_const[key()] = math.log(c)
+    T0.normal_const[key()] = math.log(c0) + math.log(c1) - 2 * math.log(c)
+    total = sum(TOT_RGSTEPS.values())
+    for direction in itertools.cycle("TXY"):
+        if sum(rgstep.values()) >= total:
+            break
+        if rgstep[direction] >= TOT_RGSTEPS[direction]:
+            continue
+        rgstep[direction] += 1
+        n = sum(rgstep.values())
+        if Tn is not None:
+            after0 = dict(T0.loc, **{direction: T0.loc[direction] // 2})
+            aftern = dict(Tn.loc, **{direction: Tn.loc[direction] // 2})
+            if sum((after0[k] - aftern[k]) ** 2 for k in after0) > 0:
+                T, T0, Tn = new_impuer_tensor_2points(T, T0, Tn, Dcut, direction)
+                T, c = normalization(T)
+                T0, c0 = normalization(T0)
+                Tn, c1 = normalization(Tn)
+                T.normal_const[key()] = math.log(c) / 2 ** n
+                T0.normal_const[key()] = math.log(c0) + math.log(c1) - 2 * math.log(c)
+                T0.loc[direction] //= 2
+                Tn.loc[direction] //= 2
+                continue
+            T, T0 = new_impuer_tensor_absorb(T, T0, Tn, Dcut, direction)
+            Tn = None
+        else:
+            T, (T0,) = new_impuer_tensor_1point(T, [T0], Dcut, direction)
+        T, c = normalization(T)
+        T0, c0 = normalization(T0)
+        T.normal_const[key()] = math.log(c) / 2 ** n
+        T0.normal_const[key()] = math.log(c0) - math.log(c)
+        T0.loc[direction] //= 2
+    lnz, (G,) = field_expected_value(T, [T0], total)
+    return T, T0, lnz, G

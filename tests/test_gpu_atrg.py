@@ -301,3 +301,73 @@ def test_atrg_one_point_matches_oracle_truncated(gpu):
     for p, r in zip(P, P_ref):
         assert abs(p - r) < 1e-9 * abs(r)
     assert [t.loc for t in T0] == [t.loc for t in T0o]
+
+
+@pytest.mark.parametrize("shape,pairs", [
+    ((4, 1, 1), [((1, 0, 0), (2, 0, 0)), ((0, 0, 0), (3, 0, 0))]),
+    ((1, 4, 1), [((0, 1, 0), (0, 2, 0))]),
+    ((1, 1, 4), [((0, 0, 2), (0, 0, 1))]),
+    ((2, 2, 2), [((0, 0, 0), (1, 1, 1))]),
+])
+def test_atrg_two_point_equals_network_contraction(gpu, shape, pairs):
+    """Device two-point flow untruncated: the correlator equals the direct
+    network contraction of the box with two different non-symmetric
+    impurities (rings of 4 fix the pairing by coordinate parity)."""
+    import itertools
+    from oracle import HOTRG_3d_QR as hq
+    from trg_backup_amd.trg import ATRG_3d_new as at
+    rng = np.random.default_rng(9)
+    T6 = models.ising3d_tensor(0.3)
+    I0 = T6 + 0.4 * rng.standard_normal(T6.shape)
+    In = 1.5 * T6 + 0.4 * rng.standard_normal(T6.shape)
+    sites = list(itertools.product(*[range(L) for L in shape]))
+    Z = hq.network_value({s: T6 for s in sites}, shape)
+    steps = {"X": shape[0].bit_length() - 1, "Y": shape[1].bit_length() - 1, "T": shape[2].bit_length() - 1}
+    at.set_rsvd_params(k_factor=3, p=0, q=2)
+    try:
+        for r0, rn in pairs:
+            Zi = hq.network_value({s: (I0 if s == r0 else In if s == rn else T6) for s in sites}, shape)
+            T, T0 = at.two_point_function(_atrg_pair(T6, 64), _atrg_pair(I0, 64, dict(zip("XYT", r0))),
+                                          _atrg_pair(In, 64, dict(zip("XYT", rn))), 64, dict(steps))
+            lnz, G = at.cal_field_expected_value(T, [T0], steps)
+            print(f"\n[atrg 2pt {shape} {r0}-{rn}] gpu={G[0]!r} network={Zi / Z!r}", flush=True)
+            assert abs(G[0] - Zi / Z) < 1e-10 * abs(Zi / Z)
+            assert abs(lnz - np.log(Z) / len(sites)) < 1e-12
+    finally:
+        at.set_rsvd_params()
+
+
+def test_atrg_two_point_matches_oracle_truncated(gpu):
+    """Truncated two-point flow (chi 8, two steps per direction, impurities at
+    (1,0,1) and (2,3,1)) against the oracle with the same sketch."""
+    from trg_backup_amd import atrg
+    from trg_backup_amd.trg import ATRG_3d_new as at
+    rng = np.random.default_rng(6)
+    T6 = models.ising3d_tensor(0.22)
+    I0 = T6 + 0.3 * rng.standard_normal(T6.shape)
+    In = T6 + 0.3 * rng.standard_normal(T6.shape)
+    r0, rn = {"X": 1, "Y": 0, "T": 1}, {"X": 2, "Y": 3, "T": 1}
+    steps = {"X": 2, "Y": 2, "T": 2}
+    chi = 8
+    oat.set_leg_swap_basis("identity")
+    oat.set_rsvd_params(k_factor=2, p=0, q=20, seed=1)
+    at.set_rsvd_params(k_factor=2, p=0, q=20, seed=1)
+    old = (oat.degeneracy_eps, atrg.degeneracy_eps)
+    oat.degeneracy_eps = atrg.degeneracy_eps = 1e-8
+    try:
+        T0o = models.to_atrg(I0, chi)
+        T0o.loc = dict(r0)
+        Tno = models.to_atrg(In, chi)
+        Tno.loc = dict(rn)
+        _, _, lnz_ref, G_ref = oat.two_point_function(models.to_atrg(T6, chi), T0o, Tno, chi, dict(steps))
+        T, T0 = at.two_point_function(_atrg_pair(T6, chi), _atrg_pair(I0, chi, r0), _atrg_pair(In, chi, rn), chi,
+                                      dict(steps))
+        lnz, G = at.cal_field_expected_value(T, [T0], steps)
+    finally:
+        oat.set_leg_swap_basis("svd")
+        oat.set_rsvd_params()
+        at.set_rsvd_params()
+        oat.degeneracy_eps, atrg.degeneracy_eps = old
+    print(f"\n[atrg 2pt chi={chi}] G {G[0]!r} vs {G_ref!r}; lnz {lnz!r} vs {lnz_ref!r}", flush=True)
+    assert abs(lnz - lnz_ref) < 1e-11 * abs(lnz_ref)
+    assert abs(G[0] - G_ref) < 1e-9 * abs(G_ref)

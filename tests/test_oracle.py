@@ -392,3 +392,38 @@ def test_hotrg3d_qr_two_point_equals_network_contraction(shape, pairs):
                                         {"Tensor": Tn, "loc": dict(zip("XYZ", rn)), "factor": {}},
                                         16, dict(steps))
         assert abs(G - Zi / Z) < 1e-11 * abs(Zi / Z), (shape, r0, rn, G, Zi / Z)
+
+
+@pytest.mark.parametrize("shape,pairs", [
+    ((4, 1, 1), [((1, 0, 0), (2, 0, 0)), ((0, 0, 0), (3, 0, 0))]),
+    ((1, 4, 1), [((0, 1, 0), (0, 2, 0))]),
+    ((1, 1, 4), [((0, 0, 2), (0, 0, 1))]),
+    ((2, 2, 2), [((0, 0, 0), (1, 1, 1))]),
+])
+def test_atrg_two_point_equals_network_contraction(shape, pairs):
+    """ATRG two-point flow (trg/ATRG_3d_new.py:1333-1489 with the one-point
+    pairing in the two-point step) untruncated: G = Z(T0 at r0, Tn at rn) / Z
+    of the box's network, with two different non-symmetric impurities (the
+    rings of 4 fix which tensor of a pair sits at the even coordinate)."""
+    import itertools
+    from oracle import HOTRG_3d_QR as hq
+    rng = np.random.default_rng(9)
+    T6 = models.ising3d_tensor(0.3)
+    I0 = T6 + 0.4 * rng.standard_normal(T6.shape)
+    In = 1.5 * T6 + 0.4 * rng.standard_normal(T6.shape)
+    sites = list(itertools.product(*[range(L) for L in shape]))
+    Z = hq.network_value({s: T6 for s in sites}, shape)
+    steps = {"X": shape[0].bit_length() - 1, "Y": shape[1].bit_length() - 1, "T": shape[2].bit_length() - 1}
+    at.set_rsvd_params(k_factor=3, p=0, q=2)
+    try:
+        for r0, rn in pairs:
+            Zi = hq.network_value({s: (I0 if s == r0 else In if s == rn else T6) for s in sites}, shape)
+            T0 = models.to_atrg(I0, 64)
+            T0.loc = dict(zip("XYT", r0))
+            Tn = models.to_atrg(In, 64)
+            Tn.loc = dict(zip("XYT", rn))
+            _, _, lnz, G = at.two_point_function(models.to_atrg(T6, 64), T0, Tn, 64, dict(steps))
+            assert abs(G - Zi / Z) < 1e-11 * abs(Zi / Z), (shape, r0, rn, G, Zi / Z)
+            assert abs(lnz - np.log(Z) / len(sites)) < 1e-12
+    finally:
+        at.set_rsvd_params()

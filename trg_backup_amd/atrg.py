@@ -285,21 +285,33 @@ def atrg_1point_step(U, s, VH, imps, Dcut, direction, prec="f64"):
     :441-482).  `imps` holds (U0, s0, VH0, odd): odd is True when the
     impurity's coordinate along `direction` is odd, and the impurity is
     then the upper tensor (A, B) of its pair, else the lower one (C, D).
-    The squeezers come from the pure pair; each impurity pair gets its own
-    leg swap and the pure pair's coarse graining.  Returns
-    ((U, s, VH), [(U0, s0, VH0), ...]) with the legs restored."""
+    Returns ((U, s, VH), [(U0, s0, VH0), ...]) with the legs restored."""
+    pairs = [((U0, s0, VH0), None) if odd else (None, (U0, s0, VH0)) for U0, s0, VH0, odd in imps]
+    return atrg_pairs_step(U, s, VH, pairs, Dcut, direction, prec)
+
+
+def atrg_pairs_step(U, s, VH, pairs, Dcut, direction, prec="f64"):
+    """One step of the pure tensor and of impurity pairs.  Each pair is
+    (upper, lower), each a (U, s, VH) or None for the pure tensor: the upper
+    tensor gives A = U and B = s VH, the lower one C = U s and D = VH.  The
+    squeezers come from the pure pair; each impurity pair gets its own leg
+    swap and the pure pair's coarse graining (trg/ATRG_3d_new.py:485-734:
+    a one-point or two-point step pairs each impurity with a pure tensor,
+    the absorb step pairs the two impurities).  Returns
+    ((U, s, VH), [(U, s, VH) per pair]) with the legs restored."""
     dt = _dt(prec)
-    A, D, B, C = _pair_operands(U, s, VH, direction, dt)
+    pure = _pair_operands(U, s, VH, direction, dt)
+    A, D, B, C = pure
     UM, sM, VMH = leg_swapping(B, C, Dcut, True, prec)
     UM, VMH = ops.cast(UM, dt), ops.cast(VMH, dt)
     Px, PX, Py, PY = squeezer(A, UM, sM, VMH, D, Dcut)
     out = []
-    for U0, s0, VH0, odd in imps:
-        A0, D0, B0, C0 = _pair_operands(U0, s0, VH0, direction, dt)
-        Ai, Bi, Ci, Di = (A0, B0, C, D) if odd else (A, B, C0, D0)
-        UMi, sMi, VMHi = leg_swapping(Bi, Ci, Dcut, False, prec)
+    for up, lo in pairs:
+        Au, _, Bu, _ = pure if up is None else _pair_operands(*up, direction, dt)
+        _, Dl, _, Cl = pure if lo is None else _pair_operands(*lo, direction, dt)
+        UMi, sMi, VMHi = leg_swapping(Bu, Cl, Dcut, False, prec)
         UMi, VMHi = ops.cast(UMi, dt), ops.cast(VMHi, dt)
-        Ui, si, VHi = coarse_graining(Ai, UMi, sMi, VMHi, Di, Px, PX, Py, PY)
+        Ui, si, VHi = coarse_graining(Au, UMi, sMi, VMHi, Dl, Px, PX, Py, PY)
         Ui, VHi = leg_transposition(Ui, VHi, "restore", direction)
         out.append((Ui, si, VHi))
     U2, s2, VH2 = coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY)

@@ -3,11 +3,13 @@
 Same names as trg/ATRG_3d_new.py:18-482 and 954-1039.  ``pure_tensor``
 takes an ATRG_Tensor (U, s, VH as NumPy arrays or GPU tensors) and returns it
 coarse grained, with normal_const filled as in :967-972.  Environment:
-DGENERACY_EPS (default 0.0, the reference requires it, :15); the per-step
-files under OUTPUT_DIR (:197-212, :736-766) are not written.  The
-one-point impurity flow (:614-734, :1043-1189) runs on the device with the
-pure tensor; the two-point and scaling-dimension flows (:485-613,
-:768-951, :1191-1489) are not on the device and raise NotImplementedError.
+DGENERACY_EPS (default 0.0, the reference requires it, :15); the flows do
+not write the per-step files under OUTPUT_DIR (:197-212, :736-766);
+save_singularvalues and cal_X write theirs when called.  The
+one-point (:614-734, :1043-1189) and two-point (:485-612, :1333-1489)
+impurity flows run on the device with the pure tensor; the scaling-dimension
+diagnostics and the one_point_function1 variant (:768-951, :1191-1331) are
+not on the device and raise NotImplementedError.
 ``set_precision("f32")`` runs the contractions in float32 on MFMA;
 decompositions stay float64.
 """
@@ -327,8 +329,6 @@ def _not_on_device(name, lines):
     return f
 
 
-new_impuer_tensor_2points = _not_on_device("new_impuer_tensor_2points", "485-555")
-new_impuer_tensor_absorb = _not_on_device("new_impuer_tensor_absorb", "557-612")
 find_max = _not_on_device("find_max", "768-816")
 find_1stto3rdmax = _not_on_device("find_1stto3rdmax", "818-852")
 cal_scaling_dimensions = _not_on_device("cal_scaling_dimensions", "854-877")
@@ -336,5 +336,91 @@ cal_coorelation = _not_on_device("cal_coorelation", "879-951")
 cal_field_expected_value1 = _not_on_device("cal_field_expected_value1", "1191-1222")
 exec_rgstep_1point_func1 = _not_on_device("exec_rgstep_1point_func1", "1224-1244")
 one_point_function1 = _not_on_device("one_point_function1", "1246-1331")
-exec_rgstep_2point_func = _not_on_device("exec_rgstep_2point_func", "1333-1417")
-two_point_function = _not_on_device("two_point_function", "1419-1489")
+
+
+# ---------------------------------------------------------------------------
+# two-point function (impurity) flow, :485-612 and :1333-1489
+# ---------------------------------------------------------------------------
+def _dev_imp(t):
+    U0, s0, VH0, _ = _dev3(t)
+    return (U0, s0, VH0)
+
+
+def new_impuer_tensor_2points(T, T0, Tn, Dcut, direction):
+    """:485-555: each impurity pairs with a pure tensor, as the upper tensor
+    when its coordinate along `direction` is odd, the lower one when even
+    (the one-point step's pairing; the reference's even branch, :519-521,
+    contracts U with s and s with VH, and :545 passes an extra first argument
+    to coarse_graining, which both fail on the shapes).  Updates T, T0, Tn
+    in place and returns them."""
+    U, s, VH, kind = _dev3(T)
+    pairs = [((_dev_imp(t), None) if t.loc[direction] % 2 == 1 else (None, _dev_imp(t))) for t in (T0, Tn)]
+    (U, s, VH), outs = atrg.atrg_pairs_step(U, s, VH, pairs, Dcut, direction, PRECISION["contract"])
+    T.update(from_device(U, kind), from_device(s, kind), from_device(VH, kind))
+    for t, (U0, s0, VH0) in zip((T0, Tn), outs):
+        t.update(from_device(U0, kind), from_device(s0, kind), from_device(VH0, kind))
+    return T, T0, Tn
+
+
+def new_impuer_tensor_absorb(T, T0, Tn, Dcut, direction):
+    """:557-612: the two impurities form one pair (the one at the odd
+    coordinate is the upper tensor) and T0 becomes their merged impurity;
+    returns (T, T0)."""
+    U, s, VH, kind = _dev3(T)
+    up, lo = (Tn, T0) if T0.loc[direction] % 2 == 0 else (T0, Tn)
+    (U, s, VH), ((U0, s0, VH0),) = atrg.atrg_pairs_step(U, s, VH, [(_dev_imp(up), _dev_imp(lo))], Dcut, direction,
+                                                       PRECISION["contract"])
+    T.update(from_device(U, kind), from_device(s, kind), from_device(VH, kind))
+    T0.update(from_device(U0, kind), from_device(s0, kind), from_device(VH0, kind))
+    return T, T0
+
+
+def exec_rgstep_2point_func(T, T0, Tn, Dcut, direction, rgstep):
+    """:1333-1417: while the impurities stay apart after the step, the
+    two-point step (T0's factor ln c0 + ln cn - 2 ln c); when they meet, the
+    absorb step (factor ln c0 - ln c, Tn becomes None); with Tn None, the
+    one-point step.  Returns (T, T0, Tn)."""
+    key = f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
+    n = sum(rgstep.values())
+    if Tn is not None:
+        after0 = dict(T0.loc, **{direction: T0.loc[direction] // 2})
+        aftern = dict(Tn.loc, **{direction: Tn.loc[direction] // 2})
+        if sqr_distance(after0, aftern) > 0:
+            T, T0, Tn = new_impuer_tensor_2points(T, T0, Tn, Dcut, direction)
+            T, c = normalization(T)
+            T0, c0 = normalization(T0)
+            Tn, c1 = normalization(Tn)
+            T.normal_const[key] = math.log(c) / 2 ** n
+            T0.normal_const[key] = math.log(c0) + math.log(c1) - 2 * math.log(c)
+            T0.loc[direction] //= 2
+            Tn.loc[direction] //= 2
+            return T, T0, Tn
+        T, T0 = new_impuer_tensor_absorb(T, T0, Tn, Dcut, direction)
+    else:
+        T, T0 = new_impuer_tensor_1point(T, T0, Dcut, direction)
+    T, c = normalization(T)
+    T0, c0 = normalization(T0)
+    T.normal_const[key] = math.log(c) / 2 ** n
+    T0.normal_const[key] = math.log(c0) - math.log(c)
+    T0.loc[direction] //= 2
+    return T, T0, None
+
+
+def two_point_function(T, T0, Tn, Dcut, TOT_RGSTEPS):
+    """:1419-1489: normalise the three tensors (T0's factor ln c0 + ln cn -
+    2 ln c), then cycle T, X, Y with exec_rgstep_2point_func.  Returns
+    (T, T0); the correlator is cal_field_expected_value(T, [T0], steps)[1][0]."""
+    rgstep = _reset_steps()
+    T, c = normalization(T)
+    T0, c0 = normalization(T0)
+    Tn, c1 = normalization(Tn)
+    T.normal_const["(0,0,0)"] = math.log(c)
+    T0.normal_const["(0,0,0)"] = math.log(c0) + math.log(c1) - 2 * math.log(c)
+    total = sum(TOT_RGSTEPS.values())
+    for direction in itertools.cycle("TXY"):
+        if sum(rgstep.values()) >= total:
+            break
+        if rgstep[direction] < TOT_RGSTEPS[direction]:
+            rgstep[direction] += 1
+            T, T0, Tn = exec_rgstep_2point_func(T, T0, Tn, Dcut, direction, rgstep)
+    return T, T0
