@@ -413,7 +413,23 @@ def gemm_sweep(args):
                 f"equal={row['bitwise_equal_to_config_4']}")
         del a, b
         torch.cuda.empty_cache()
-    print(json.dumps({"gemm_sweep": rows}), flush=True)
+    # library reference point: torch.bmm (hipBLASLt) on the same bf16 product, contiguous operands,
+    # bf16 output in a plain row-major destination
+    del C, Cref
+    torch.cuda.empty_cache()
+    a, b = A.to(torch.bfloat16), B.transpose(1, 2).contiguous().to(torch.bfloat16)
+    del A, B
+    c = torch.bmm(a, b)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(3):
+        torch.bmm(a, b, out=c)
+    ev[1].record()
+    ev[1].synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / 3
+    lib_row = {"compute": "torch.bmm bf16 (hipBLASLt), plain layout", "ms": ms, "tflops": flops / ms / 1e9}
+    log(f"torch.bmm bf16 {ms:8.2f} ms {lib_row['tflops']:7.1f} TF/s")
+    print(json.dumps({"gemm_sweep": rows, "library": lib_row}), flush=True)
     return 0 if all(r["bitwise_equal_to_config_4"] for r in rows) else 1
 
 
