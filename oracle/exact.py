@@ -86,3 +86,14 @@ def brute_corr_ising(beta, shape, a, b):
     w = beta * E
     p = np.exp(w - np.max(w))
     return float(np.sum(p * states[:, ia] * states[:, ib]) / np.sum(p))
+
+
+def brute_ising_field(beta, h, shape=(2, 2, 2)):
+    """(ln Z / V, <s>) of the periodic Ising box ``shape`` with coupling beta
+    and field h (weight exp(beta sum s s' + h sum s)), by enumeration; same
+    bond convention as brute_lnZ_ising3d."""
+    V, bonds = _torus_bonds(shape)
+    states = ((np.arange(2 ** V)[:, None] >> np.arange(V)[None, :]) & 1) * 2 - 1
+    w = beta * np.sum(states[:, bonds[:, 0]] * states[:, bonds[:, 1]], axis=1) + h * np.sum(states, axis=1)
+    p = np.exp(w - np.max(w))
+    return _lnsumexp(w) / V, float(np.sum(p * states[:, 0]) / np.sum(p))

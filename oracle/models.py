@@ -63,3 +63,16 @@ def zero_pad(T, D):
     out = np.zeros((D,) * T.ndim, dtype=T.dtype)
     out[tuple(slice(0, n) for n in T.shape)] = T
     return out
+
+
+def ising3d_field_tensors(beta, h):
+    """measurement/Ising_model_3d.py:15-79 (real form): M = exp(beta s s') split
+    by its SVD as u sqrt(s), sqrt(s) vh; site weight exp(h s); the impurity
+    carries one more factor s.  Returns (T, Timp), legs (x,X,y,Y,t,T)."""
+    s = np.array([1.0, -1.0])
+    u, sv, vh = np.linalg.svd(np.exp(beta * np.outer(s, s)))
+    us, svh = u * np.sqrt(sv)[None, :], np.sqrt(sv)[:, None] * vh
+    site = np.exp(h * s)
+    T = np.einsum("a,xa,aX,ya,aY,ta,aT->xXyYtT", site, svh, us, svh, us, svh, us)
+    Timp = np.einsum("a,xa,aX,ya,aY,ta,aT->xXyYtT", s * site, svh, us, svh, us, svh, us)
+    return T, Timp

@@ -427,3 +427,23 @@ def test_atrg_two_point_equals_network_contraction(shape, pairs):
             assert abs(lnz - np.log(Z) / len(sites)) < 1e-12
     finally:
         at.set_rsvd_params()
+
+
+@pytest.mark.parametrize("h", [0.0, 0.15])
+def test_ising3d_field_tensors_against_enumeration(h):
+    """The field form of the 3D Ising initial tensors (measurement/Ising_model_3d.py:15-79):
+    untruncated HOTRG_3d_QR on the 2x2x2 box gives ln Z/V and the one-point value
+    <s> = P of the impurity flow equal to the enumeration with field h."""
+    beta = 0.2
+    T, Timp = models.ising3d_field_tensors(beta, h)
+    lnz_ref, m_ref = exact.brute_ising_field(beta, h)
+    lnz, _ = hq.ln_Z_over_V(T, 16, {"X": 1, "Y": 1, "Z": 1}, "QR")
+    assert abs(lnz - lnz_ref) < 1e-12
+    if h == 0.0:
+        assert abs(m_ref) < 1e-14
+        assert abs(lnz - exact.brute_lnZ_ising3d(beta)) < 1e-12
+        return
+    _, _, P = hq.one_point_function({"Tensor": T, "loc": {}, "factor": {}},
+                                    {"Tensor": Timp, "loc": {"X": 1, "Y": 0, "Z": 1}, "factor": {}}, 16,
+                                    {"X": 1, "Y": 1, "Z": 1}, "QR")
+    assert abs(P - m_ref) < 1e-11 * abs(m_ref)

@@ -37,6 +37,8 @@ SURFACE = {
     "utility.randomized_svd": ["rsvd", "rsvd2", "__power_iteration__", "__QR_iteration__",
                                "rsvd_for_3dATRG_tensor_init", "rsvd_for_3dATRG_tensor_inte"],
     "tensor_class.tensor_class": ["HOTRG_Tensor", "Tensor", "ATRG_Tensor"],
+    "measurement.Ising_model_3d": ["init_tensor", "init_tensor_imp", "ln_Z_over_V"],
+    "tensor_init.Potts_model": ["infinite_density"],
 }
 
 
@@ -52,7 +54,7 @@ def aliases():
 
 def test_aliases_resolve_to_the_gpu_modules(aliases):
     import importlib
-    assert set(aliases) == set(SURFACE) | {"trg", "utility", "tensor_class"}
+    assert set(aliases) == set(SURFACE) | {"trg", "utility", "tensor_class", "measurement", "tensor_init"}
     for alias in SURFACE:
         mod = importlib.import_module(alias)
         assert mod.__name__ == "trg_backup_amd." + alias

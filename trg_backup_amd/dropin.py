@@ -26,6 +26,10 @@ ALIASES = {
     "utility.randomized_svd": "trg_backup_amd.utility.randomized_svd",
     "tensor_class": "trg_backup_amd.tensor_class",
     "tensor_class.tensor_class": "trg_backup_amd.tensor_class.tensor_class",
+    "measurement": "trg_backup_amd.measurement",
+    "measurement.Ising_model_3d": "trg_backup_amd.measurement.Ising_model_3d",
+    "tensor_init": "trg_backup_amd.tensor_init",
+    "tensor_init.Potts_model": "trg_backup_amd.tensor_init.Potts_model",
 }
 
 
