@@ -57,9 +57,11 @@ def leg_transposition(U, VH, do_what="transpose", direction="t"):
 # ---------------------------------------------------------------------------
 # leg swapping (:67-235)
 # ---------------------------------------------------------------------------
-def rsvd_for_ATRG_leg_swapping(Bt, Ct, chi_k, p, q, eps, seed):
+def rsvd_for_ATRG_leg_swapping(Bt, Ct, chi_k, p, q, eps, seed, keep=None):
     """Randomized SVD of M_{(i,g),(j,b)} = sum_t Bt_{b,i,t} Ct_{t,j,g}
-    (:67-140).  Returns UM (i,g,k), sM (k,), VMH (k,j,b) as float64."""
+    (:67-140).  Returns UM (i,g,k), sM (k,), VMH (k,j,b) as float64.
+    `keep`: the number of singular triplets the caller keeps at most (the
+    sketch may shrink to it, see below)."""
     nb, ni, nt = Bt.shape
     _, nj, ng = Ct.shape
     l = chi_k + p
@@ -91,20 +93,41 @@ def rsvd_for_ATRG_leg_swapping(Bt, Ct, chi_k, p, q, eps, seed):
         u, s, vh = linalg.svd_thin(M[None])
         UM, sM, VMH = u[0], s[0], vh[0]
     else:
-        Y = ops.gaussian((nj, nb, l), seed=seed, device=Bt.device)[..., :ls]
-        Y = ops.contiguous(Y, dt)
-        Q = linalg.orthonormalize(apply_M(Y).reshape(1, rows_i, ls))[0]
-        for _ in range(q):
-            Z = apply_MT(ops.cast(Q.view(ni, ng, ls), dt))
-            Q2 = linalg.orthonormalize(_f64(Z).reshape(1, rows_j, ls))[0]
-            Y = apply_M(ops.cast(Q2.view(nj, nb, ls), dt))
-            Q = linalg.orthonormalize(_f64(Y).reshape(1, rows_i, ls))[0]
-        # X^T = M^T Q : (j,b,l), then the thin SVD of X (l x rows_j)
-        XT = _f64(apply_MT(ops.cast(Q.view(ni, ng, ls), dt))).reshape(1, rows_j, ls)
-        v, s, uh = linalg.svd_thin(XT)       # X^T = v s uh  ->  X = uh^T s v^T
-        UM = ops.gemm(Q, ops.contiguous(uh[0].transpose(0, 1)))
-        sM = s[0]
-        VMH = ops.contiguous(v[0].transpose(0, 1))
+        def sketch(ls, perturb):
+            infos = []
+
+            def orth(X):
+                Q, info = linalg.orthonormalize(X, perturb=perturb, return_info=True)
+                infos.append(info)
+                return Q[0]
+            Y = ops.gaussian((nj, nb, l), seed=seed, device=Bt.device)[..., :ls]
+            Y = ops.contiguous(Y, dt)
+            Q = orth(apply_M(Y).reshape(1, rows_i, ls))
+            for _ in range(q):
+                Z = apply_MT(ops.cast(Q.view(ni, ng, ls), dt))
+                Q2 = orth(_f64(Z).reshape(1, rows_j, ls))
+                Y = apply_M(ops.cast(Q2.view(nj, nb, ls), dt))
+                Q = orth(_f64(Y).reshape(1, rows_i, ls))
+            # X^T = M^T Q : (j,b,l), then the thin SVD of X (l x rows_j)
+            XT = _f64(apply_MT(ops.cast(Q.view(ni, ng, ls), dt))).reshape(1, rows_j, ls)
+            v, s, uh = linalg.svd_thin(XT)       # X^T = v s uh  ->  X = uh^T s v^T
+            bad = int(torch.stack(infos).max().item()) != 0
+            return ops.gemm(Q, ops.contiguous(uh[0].transpose(0, 1))), s[0], ops.contiguous(v[0].transpose(0, 1)), bad
+
+        # A sketch wider than the rank of M (small systems, or a bond that
+        # is rank deficient) breaks the Cholesky of the range finder down at
+        # the rounding level.  Narrow the sketch towards the number of
+        # triplets kept; only if that still fails, repair the blocks with a
+        # perturbation at 1e-10 of their largest entry.
+        floor = max(1, min(ls, keep if keep else ls))
+        UM, sM, VMH, bad = sketch(ls, linalg.PERTURB)
+        while bad and ls > floor:
+            ls = max(floor, ls // 2)
+            UM, sM, VMH, bad = sketch(ls, linalg.PERTURB)
+        if bad:
+            UM, sM, VMH, bad = sketch(ls, 1e-10)
+            if bad:
+                raise TrgError("leg-swap range finder: Cholesky breakdown on a rank-deficient sketch")
     k = min(linalg.count_above(sM, eps), chi_k)
     k = max(k, 1)
     UM = ops.contiguous(UM[:, :k]).view(ni, ng, k)
@@ -127,7 +150,7 @@ def leg_swapping(B, C, Dcut, ispure=True, prec="f64"):
     Ct = ops.permute(C, (0, 3, 1, 2)).view(nT0, nj, nX0 * nY0)      # (t, j, g)
     q = Dcut if RSVD["q"] is None else RSVD["q"]
     UM, sM, VMH = rsvd_for_ATRG_leg_swapping(Bt, Ct, RSVD["k_factor"] * chi_k, RSVD["p"], q, truncate_eps,
-                                             RSVD["seed"])
+                                             RSVD["seed"], keep=chi_k)
     chi, _ = linalg.truncate(sM, chi_k, degeneracy_eps, truncate_eps)
     UM = ops.contiguous(UM[:, :, :chi]).view(ni, nX0, nY0, chi)
     VMH = ops.contiguous(VMH[:chi]).view(chi, nj, nx1, ny1)

@@ -46,7 +46,7 @@ def _fits_small(m, n):
     return n <= m <= 256 and m * n * 8 + 16384 <= 163840
 
 
-def orthonormalize(X, perturb=PERTURB, stream_id=7, passes=3):
+def orthonormalize(X, perturb=PERTURB, stream_id=7, passes=3, return_info=False):
     """Orthonormal basis (B, m, l) of the columns of X (B, m, l), l <= m.
     A relative Gaussian perturbation keeps rank-deficient blocks of full
     column rank (directions below perturb * max|X| are noise either way).
@@ -67,7 +67,9 @@ def orthonormalize(X, perturb=PERTURB, stream_id=7, passes=3):
     info = torch.empty(B, dtype=torch.int32, device=X.device)
     call("trg_orthonormalize_f64", B, m, l, ptr(X), m * l, ptr(Q), ptr(ws), ws.numel() * 8, passes, ptr(info),
          stream_ptr())
-    return Q
+    # info[b] != 0: a Cholesky pivot fell to the rounding level (the block is
+    # rank deficient beyond what the perturbation repairs)
+    return (Q, info) if return_info else Q
 
 
 def svd_thin(A):
