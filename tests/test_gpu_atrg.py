@@ -371,3 +371,30 @@ def test_atrg_two_point_matches_oracle_truncated(gpu):
     print(f"\n[atrg 2pt chi={chi}] G {G[0]!r} vs {G_ref!r}; lnz {lnz!r} vs {lnz_ref!r}", flush=True)
     assert abs(lnz - lnz_ref) < 1e-11 * abs(lnz_ref)
     assert abs(G[0] - G_ref) < 1e-9 * abs(G_ref)
+
+
+def test_atrg_init_and_inte_rsvds_exact_when_sketch_covers_rank(gpu):
+    """utility/randomized_svd.py:100-226 (rsvd_for_3dATRG_tensor_init / _inte):
+    with a sketch at least the rank of the product, the singular values equal
+    the dense SVD of the product and U s VH rebuilds it.  The sketches are
+    wider than the ranks (12 columns for rank 5), which the range finder
+    repairs with a 1e-10 perturbation, hence the 1e-9 tolerance."""
+    from trg_backup_amd.utility import randomized_svd as rs
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((3, 3, 3, 5))
+    B = rng.standard_normal((5, 3, 3, 3))
+    M = np.einsum("abci,idef->abcdef", A, B).reshape(27, 27)
+    s_ref = np.linalg.svd(M, compute_uv=False)
+    u, s, vh = rs.rsvd_for_3dATRG_tensor_init(A, B, 8, n_oversamples=4, n_power_iter=2)
+    assert np.allclose(s[:5], s_ref[:5], rtol=1e-9, atol=1e-9 * s_ref[0])
+    Mr = np.einsum("abcj,j,jdef->abcdef", u[..., :5], s[:5], vh[:5]).reshape(27, 27)
+    assert np.abs(Mr - M).max() < 1e-9 * np.abs(M).max()
+    Bt = rng.standard_normal((3, 2, 3, 3))      # B_{idbc}
+    Ct = rng.standard_normal((2, 3, 3, 3))      # C_{djka}
+    M2 = np.einsum("idbc,djka->ijkabc", Bt, Ct).reshape(27, 27)
+    s2_ref = np.linalg.svd(M2, compute_uv=False)
+    r2 = int((s2_ref > 1e-12 * s2_ref[0]).sum())
+    u2, s2, vh2 = rs.rsvd_for_3dATRG_tensor_inte(Bt, Ct, 20, n_oversamples=4, n_power_iter=2)
+    assert np.allclose(s2[:r2], s2_ref[:r2], rtol=1e-9, atol=1e-9 * s2_ref[0])
+    M2r = np.einsum("ijkl,l,labc->ijkabc", u2[..., :r2], s2[:r2], vh2[:r2]).reshape(27, 27)
+    assert np.abs(M2r - M2).max() < 1e-9 * np.abs(M2).max()

@@ -9,6 +9,7 @@ Householder QR); the small SVD is the libtrg one-sided Jacobi SVD.
 import torch
 
 from .. import linalg, ops
+from .._lib import TrgError
 from .._io import from_device, to_device
 from ..contract import contract
 
@@ -31,7 +32,18 @@ def _finish(Q, Amul_T, k, kind):
 
 
 def _orth(Y):
-    return linalg.orthonormalize(ops.cast(Y, torch.float64)[None])[0]
+    """Orthonormal basis of the sketch's columns.  A sketch wider than the
+    rank of the operator can break the range finder's Cholesky down at the
+    rounding level (flagged by the library); it is then repaired with a
+    perturbation at 1e-10 of its largest entry (directions that small are
+    noise either way)."""
+    X = ops.cast(Y, torch.float64)[None]
+    Q, info = linalg.orthonormalize(X, return_info=True)
+    if int(info.max().item()) != 0:
+        Q, info = linalg.orthonormalize(X, perturb=1e-10, return_info=True)
+        if int(info.max().item()) != 0:
+            raise TrgError("randomized SVD: Cholesky breakdown on a rank-deficient sketch")
+    return Q[0]
 
 
 def rsvd(A, k, n_oversamples=0, n_power_iter=0, iterator="QR", seed=1234):
