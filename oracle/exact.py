@@ -97,3 +97,15 @@ def brute_ising_field(beta, h, shape=(2, 2, 2)):
     w = beta * np.sum(states[:, bonds[:, 0]] * states[:, bonds[:, 1]], axis=1) + h * np.sum(states, axis=1)
     p = np.exp(w - np.max(w))
     return _lnsumexp(w) / V, float(np.sum(p * states[:, 0]) / np.sum(p))
+
+
+def brute_corr_ising_field(beta, h, shape, a, b):
+    """<s_a s_b> of the periodic Ising box with field h (weight exp(beta sum s s' +
+    h sum s)), by enumeration; same bond convention as brute_lnZ_ising3d."""
+    V, bonds = _torus_bonds(shape)
+    sites = list(itertools.product(*[range(L) for L in shape]))
+    ia, ib = sites.index(tuple(a)), sites.index(tuple(b))
+    states = ((np.arange(2 ** V)[:, None] >> np.arange(V)[None, :]) & 1) * 2 - 1
+    w = beta * np.sum(states[:, bonds[:, 0]] * states[:, bonds[:, 1]], axis=1) + h * np.sum(states, axis=1)
+    p = np.exp(w - np.max(w))
+    return float(np.sum(p * states[:, ia] * states[:, ib]) / np.sum(p))

@@ -66,3 +66,29 @@ def test_potts_infinite_density_init(gpu):
     assert abs(va - ref2) < 1e-12, (va, ref2)
     with pytest.raises(NotImplementedError):
         infinite_density(3, 64, 3, k, 0.1, 0.0).cal_HOTRG_init_tensor()
+
+
+@pytest.mark.parametrize("rgscheme", ["hotrg", "atrg"])
+def test_ising3d_magnetization_and_internal_energy(gpu, rgscheme, monkeypatch):
+    """measurement/Ising_model_3d.py magnetization and internal_energy on the
+    device, untruncated on the 2x2x2 box with a field: |<s>| and the
+    nearest-neighbour <s_0 s_1> along t equal the enumeration."""
+    from trg_backup_amd.measurement import Ising_model_3d as im
+    beta, h = 0.2, 0.15
+    lnz_ref, m_ref = exact.brute_ising_field(beta, h)
+    c_ref = exact.brute_corr_ising_field(beta, h, (2, 2, 2), (0, 0, 0), (0, 0, 1))
+    monkeypatch.setenv("RGSCHEME", rgscheme)
+    Dcut = 16 if rgscheme == "hotrg" else 64
+    if rgscheme == "atrg":
+        from trg_backup_amd.trg import ATRG_3d_new as at
+        at.set_rsvd_params(k_factor=3, p=0, q=2)
+    try:
+        lnz, m = im.magnetization(beta, h, Dcut, 1, 1, 1)
+        lnz2, c = im.internal_energy(beta, h, Dcut, 1, 1, 1)
+    finally:
+        if rgscheme == "atrg":
+            at.set_rsvd_params()
+    print(f"\n[{rgscheme}] <s>={m!r} ({m_ref!r}) <s0 s1>={c!r} ({c_ref!r})", flush=True)
+    assert abs(lnz - lnz_ref) < 1e-12 and abs(lnz2 - lnz_ref) < 1e-12
+    assert abs(m - abs(m_ref)) < 1e-10 * abs(m_ref)
+    assert abs(c - c_ref) < 1e-10 * abs(c_ref)

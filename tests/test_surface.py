@@ -37,7 +37,8 @@ SURFACE = {
     "utility.randomized_svd": ["rsvd", "rsvd2", "__power_iteration__", "__QR_iteration__",
                                "rsvd_for_3dATRG_tensor_init", "rsvd_for_3dATRG_tensor_inte"],
     "tensor_class.tensor_class": ["HOTRG_Tensor", "Tensor", "ATRG_Tensor"],
-    "measurement.Ising_model_3d": ["init_tensor", "init_tensor_imp", "ln_Z_over_V"],
+    "measurement.Ising_model_3d": ["init_tensor", "init_tensor_imp", "ln_Z_over_V", "magnetization",
+                                   "internal_energy"],
     "tensor_init.Potts_model": ["infinite_density"],
 }
 

@@ -79,3 +79,59 @@ def ln_Z_over_V(beta, h, Dcut, XLOOPS, YLOOPS, TLOOPS, gilt_eps=0):
         T = a.pure_tensor(init_tensor(beta, h, Dcut, rgscheme), Dcut, {"X": XLOOPS, "Y": YLOOPS, "T": TLOOPS})
         return float(sum(T.normal_const.values())) + np.log(float(T.trace())) / V
     raise ValueError("Only support hotrg")
+
+
+def _scheme():
+    return os.environ.get("RGSCHEME", "hotrg")
+
+
+def magnetization(beta, h, Dcut, XLOOPS, YLOOPS, TLOOPS, gilt_eps=0):
+    """:140-204: (ln Z/V, |<s>|) from the one-point flow of the spin impurity
+    at the origin (HOTRG_3d_QR or ATRG_3d_new by RGSCHEME)."""
+    V = 2 ** (XLOOPS + YLOOPS + TLOOPS)
+    rgscheme = _scheme()
+    if rgscheme == "hotrg":
+        from ..trg import HOTRG_3d_QR as q
+        T, Ti = init_tensor_imp(beta, h, Dcut, rgscheme)
+        steps = {"X": XLOOPS, "Y": YLOOPS, "Z": TLOOPS}
+        T, T0 = q.one_point_function({"Tensor": T, "loc": {}, "factor": {}},
+                                     {"Tensor": Ti, "loc": {"X": 0, "Y": 0, "Z": 0}, "factor": {}}, Dcut, steps)
+        lnz, P = q.cal_field_expected_value(T, [T0], steps, False)
+        return lnz, abs(P[0])
+    if rgscheme == "atrg":
+        from ..trg import ATRG_3d_new as a
+        T, Ti = init_tensor_imp(beta, h, Dcut, rgscheme)
+        steps = {"X": XLOOPS, "Y": YLOOPS, "T": TLOOPS}
+        T, (T0,) = a.one_point_function(T, [Ti], Dcut, steps)
+        lnz, P = a.cal_field_expected_value(T, [T0], steps)
+        return lnz, abs(P[0])
+    raise ValueError("Only support hotrg")
+
+
+def internal_energy(beta, h, Dcut, XLOOPS, YLOOPS, TLOOPS, gilt_eps=0):
+    """:207-268: (ln Z/V, <s_0 s_1>) for the nearest neighbours (0,0,0) and
+    (0,0,1) along t, from the two-point flow.  The flow normalises each
+    impurity by |Tr|, which vanishes for the bare spin at h = 0 (the
+    reference divides by it as well): use a non-zero field."""
+    V = 2 ** (XLOOPS + YLOOPS + TLOOPS)
+    rgscheme = _scheme()
+    if rgscheme == "hotrg":
+        from ..trg import HOTRG_3d_QR as q
+        T, Ti = init_tensor_imp(beta, h, Dcut, rgscheme)
+        steps = {"X": XLOOPS, "Y": YLOOPS, "Z": TLOOPS}
+        T, T0 = q.two_point_function({"Tensor": T, "loc": {}, "factor": {}},
+                                     {"Tensor": Ti, "loc": {"X": 0, "Y": 0, "Z": 0}, "factor": {}},
+                                     {"Tensor": Ti.clone(), "loc": {"X": 0, "Y": 0, "Z": 1}, "factor": {}}, Dcut,
+                                     steps)
+        lnz, G = q.cal_field_expected_value(T, [T0], steps, False)
+        return lnz, G[0]
+    if rgscheme == "atrg":
+        from ..trg import ATRG_3d_new as a
+        T, T0 = init_tensor_imp(beta, h, Dcut, rgscheme)
+        _, Tn = init_tensor_imp(beta, h, Dcut, rgscheme)
+        T0.loc, Tn.loc = {"X": 0, "Y": 0, "T": 0}, {"X": 0, "Y": 0, "T": 1}
+        steps = {"X": XLOOPS, "Y": YLOOPS, "T": TLOOPS}
+        T, T0 = a.two_point_function(T, T0, Tn, Dcut, steps)
+        lnz, G = a.cal_field_expected_value(T, [T0], steps)
+        return lnz, G[0]
+    raise ValueError("Only support hotrg")
