@@ -768,9 +768,9 @@ gemm_bf16in_kernel(GemmArgs g) {
 // into registers one iteration earlier) into the stage nobody reads, then
 // issues the global loads of slice t + 2, then runs the MFMAs of slice t from
 // the other stage, then one barrier: the global latency is hidden behind a
-// whole iteration of MFMAs.  Loads are unconditional (indices clamped into the
-// operand, out-of-range vectors replaced by zeros afterwards) and the last
-// slice is peeled off, so the loop body is a single basic block that the
+// whole iteration of MFMAs.  Loads are buffer loads whose out-of-range
+// vectors come back as zeros, and the last slice is peeled off, so the loop
+// body is a single basic block that the
 // compiler can interleave; SCHED = 1 also asks its instruction-group
 // scheduler for an explicit pattern (LDS reads of the first k step, then
 // each MFMA followed by a share of the LDS reads, conversion VALU work, LDS
