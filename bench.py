@@ -480,8 +480,9 @@ def schedule(nsteps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(independent couplings shard trivially across GPUs, see DESIGN.md)"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus above 1 is not supported "
+                                   "(independent couplings shard across GPUs as whole flows, with no "
+                                   "exchange inside a step)"}))
         return 2
     wl = args.workload
     args.chi = args.chi or {"hotrg2d": 64, "hotrg3d": 16, "hotrg3d_qr": 16, "atrg3d": 64}.get(wl, 64)
