@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--atrg-q", type=int, default=2)
     ap.add_argument("--squeezer", default="QR", choices=["original", "QR", "fix"], help="hotrg3d_qr squeezer")
     ap.add_argument("--atrg-prec", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--split6-env-from", type=int, default=6,
+                    help="three-piece split-bf16 environments (fp32 accuracy) after this step (-1: never)")
     ap.add_argument("--split6-merge-from", type=int, default=6,
                     help="bf16x6 merge (three-piece split-bf16, fp32 accuracy) after this step (-1: never)")
     ap.add_argument("--split-merge-from", type=int, default=10, help="bf16x3 merge after this step (-1: never)")
@@ -514,7 +516,7 @@ def main():
     opt = lambda v: None if v < 0 else v
     policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32, bf16_env_from=opt(args.bf16_env_from),
                           split_merge_from=opt(args.split_merge_from), bf16_merge_from=opt(args.bf16_merge_from),
-                          split6_merge_from=opt(args.split6_merge_from))
+                          split6_merge_from=opt(args.split6_merge_from), split6_env_from=opt(args.split6_env_from))
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     T0 = torch.stack([models.initial_tensor("ising2d", b, dev) for b in betas])
 

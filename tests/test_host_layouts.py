@@ -47,3 +47,19 @@ def test_merge2d_chain_layouts(monkeypatch, shape, compute):
     # would show as an O(1) error, rounding as ~1e-2 relative)
     tol = 3e-2 * ref.abs().max().item() if compute == "bf16" and (Dy * D) % 8 == 0 else 1e-10
     assert (out - ref).abs().max().item() < tol
+
+
+def test_precision_policy_tiers():
+    """The per-step precision schedule: float64 through fp64_steps, then the
+    three-piece split (fp32 accuracy) for environments and merges, the
+    two-piece split merge, and bf16 last; environments switch to bf16 after
+    their own step."""
+    pol = hotrg.Policy(fp64_steps=6, low=hotrg.FP32, split6_env_from=6, bf16_env_from=12,
+                       split6_merge_from=6, split_merge_from=10, bf16_merge_from=22)
+    assert pol(6) is hotrg.FP64
+    assert (pol(7).env, pol(7).merge) == ("bf16x6", "bf16x6")
+    assert (pol(11).env, pol(11).merge) == ("bf16x6", "bf16x3")
+    assert (pol(13).env, pol(13).merge) == ("bf16", "bf16x3")
+    assert (pol(23).env, pol(23).merge) == ("bf16", "bf16")
+    assert pol(7).env_compute() == "bf16x6" and pol(7).store == torch.float32
+    assert "bf16x6 env after step 6" in pol.describe()

@@ -63,9 +63,10 @@ class Policy:
     early steps run in float64 and later ones in lower precision."""
 
     def __init__(self, fp64_steps=10**9, low=FP32, bf16_env_from=None, bf16_merge_from=None,
-                 split_merge_from=None, split6_merge_from=None):
+                 split_merge_from=None, split6_merge_from=None, split6_env_from=None):
         self.fp64_steps = fp64_steps
         self.low = low
+        self.split6_env_from = split6_env_from
         self.bf16_env_from = bf16_env_from
         self.bf16_merge_from = bf16_merge_from
         self.split_merge_from = split_merge_from
@@ -75,6 +76,8 @@ class Policy:
         if step <= self.fp64_steps:
             return FP64
         env, merge = self.low.env, self.low.merge
+        if self.split6_env_from is not None and step > self.split6_env_from:
+            env = "bf16x6"
         if self.bf16_env_from is not None and step > self.bf16_env_from:
             env = "bf16"
         if self.split6_merge_from is not None and step > self.split6_merge_from:
@@ -89,6 +92,8 @@ class Policy:
 
     def describe(self):
         s = f"fp64 for steps <= {self.fp64_steps}, then {self.low.store} env={self.low.env} merge={self.low.merge}"
+        if self.split6_env_from is not None:
+            s += f", bf16x6 env after step {self.split6_env_from}"
         if self.bf16_env_from is not None:
             s += f", bf16 env after step {self.bf16_env_from}"
         if self.split6_merge_from is not None:
