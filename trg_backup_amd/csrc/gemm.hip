@@ -1151,6 +1151,7 @@ static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmAr
             case 11: return launch_bf16<256, 128, 32, 64, 64, false, true, true>(g, batch, amc, bnc, s);
             case 12: return launch_bf16p<256, 256, 32, 128, 64, 0, false, 0>(g, batch, amc, bnc, s);
             case 13: return launch_bf16p<256, 256, 32, 128, 64, 0, false, 1>(g, batch, amc, bnc, s);
+            case 14: return launch_bf16p<64, 256, 32, 64, 64, 0, false, 2>(g, batch, amc, bnc, s);
             default: return TRG_BAD_ARG;
         }
     }
@@ -1171,6 +1172,7 @@ static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmAr
             case 11: return launch_bf16p<256, 256, 32, 128, 64, 1, false, 2>(g, batch, amc, bnc, s);
             case 12: return launch_bf16p<256, 256, 32, 128, 128, 1, false, 0>(g, batch, amc, bnc, s);
             case 13: return launch_bf16p<256, 256, 32, 128, 128, 1, false, 2>(g, batch, amc, bnc, s);
+            case 14: return launch_bf16p<64, 256, 32, 64, 64, 1, false, 2>(g, batch, amc, bnc, s);
             default: return TRG_BAD_ARG;
         }
     }
@@ -1188,6 +1190,7 @@ static int gemm_dispatch(int dtype, int c_dtype, int compute, int config, GemmAr
             case 8: return launch_bf16p<256, 256, 16, 128, 64, 2, false, 1>(g, batch, amc, bnc, s);
             case 9: return launch_bf16p<256, 256, 16, 128, 64, 2, false, 2>(g, batch, amc, bnc, s);
             case 10: return launch_bf16p<256, 256, 16, 128, 128, 2, false, 0>(g, batch, amc, bnc, s);
+            case 11: return launch_bf16p<64, 256, 16, 64, 64, 2, false, 2>(g, batch, amc, bnc, s);
             default: return TRG_BAD_ARG;
         }
     }
@@ -1239,9 +1242,9 @@ extern "C" int trg_gemm_config_count(int dtype, int c_dtype, int compute) {
     if (dtype == TRG_F32 && c_dtype == TRG_BF16 && compute == TRG_BF16) return 3;
     if (c_dtype != dtype) return 0;
     if (dtype == TRG_F32 && compute == TRG_F32) return 8;
-    if (dtype == TRG_F32 && compute == TRG_BF16) return 14;
-    if (dtype == TRG_F32 && compute == TRG_BF16X3) return 14;
-    if (dtype == TRG_F32 && compute == TRG_BF16X6) return 11;
+    if (dtype == TRG_F32 && compute == TRG_BF16) return 15;
+    if (dtype == TRG_F32 && compute == TRG_BF16X3) return 15;
+    if (dtype == TRG_F32 && compute == TRG_BF16X6) return 12;
     if (dtype == TRG_F64 && compute == TRG_F64) return 4;
     return 0;
 }
@@ -1253,12 +1256,13 @@ extern "C" const char *trg_gemm_config_name(int dtype, int c_dtype, int compute,
                                "128x256x32 w64x128", "128x128x64 w64x64", "256x128x64 w128x64",
                                "256x256x32 w128x64", "256x256x16 w128x64", "128x256x32 w64x64",
                                "256x256x32 w128x64 db", "256x128x32 w64x64 db", "256x256x32 w128x64 pipe",
-                               "256x256x32 w128x64 pipe sched"};
+                               "256x256x32 w128x64 pipe sched", "64x256x32 w64x64 pipe phased"};
     static const char *bx3[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "128x256x32 w64x64",
                                 "256x256x32 w128x64", "256x128x32 w128x64", "256x128x32 w64x64 db",
                                 "128x256x32 w64x64 db", "256x256x32 w128x64 pipe", "256x256x32 w128x64 pipe sched",
                                 "256x128x32 w64x64 pipe sched", "256x256x32 w128x64 pipe phased",
-                                "256x256x32 w128x128 pipe", "256x256x32 w128x128 pipe phased"};
+                                "256x256x32 w128x128 pipe", "256x256x32 w128x128 pipe phased",
+                                "64x256x32 w64x64 pipe phased"};
     static const char *bin[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "256x256x32 w128x64",
                                 "256x256x64 w128x64", "256x256x32 w128x64 db", "256x128x64 w64x64 db",
                                 "128x256x32 w64x64", "256x256x64 w128x64 db", "256x256x128 w128x64",
@@ -1268,7 +1272,8 @@ extern "C" const char *trg_gemm_config_name(int dtype, int c_dtype, int compute,
     static const char *bx6[] = {"128x128x32 w64x64", "64x64x32 w32x32", "256x128x32 w64x64", "128x256x32 w64x64",
                                 "256x256x32 w128x64", "256x128x32 w128x64", "128x128x32 w64x64 pipe sched",
                                 "256x256x16 w128x64 pipe", "256x256x16 w128x64 pipe sched",
-                                "256x256x16 w128x64 pipe phased", "256x256x16 w128x128 pipe"};
+                                "256x256x16 w128x64 pipe phased", "256x256x16 w128x128 pipe",
+                                "64x256x16 w64x64 pipe phased"};
     static const char *f64[] = {"128x128x16 w64x64", "64x64x16 w32x32", "128x64x16 w64x32", "64x128x16 w32x64"};
     if (config < 0 || config >= trg_gemm_config_count(dtype, c_dtype, compute)) return "";
     if (dtype == TRG_BF16) return bin[config];

@@ -249,8 +249,12 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
         Z2p = torch.empty(B * Dy * chi_a * Dy * nc * Dx, dtype=T1.dtype, device=T1.device)
         gemm(Z2a, Tp, Z2p, compute=compute, tag="merge2d_z2",
              out_layout=([(nc, Dx), (Dy * chi_a, Dy * nc * Dx)], [(Dy, nc * Dx), (Dx, 1)]))
-        # out^T[j,(k,i,l)] += Ub^T Z2^T: rows j are contiguous in out (vector epilogue)
-        gemm(Ubm[:, c0 * Dx:c1 * Dx, :].transpose(1, 2), Z2p.view(B, Dy * chi_a * Dy, nc * Dx).transpose(1, 2),
+        # out^T[j,(k,i,l)] += Ub^T Z2^T: rows j are contiguous in out (vector epilogue).  On the bf16
+        # MFMA the small Ub^T slice is copied K-contiguous, so both operands stream as 16-byte
+        # vectors through the pipelined kernel (same product bits in any layout)
+        UbT = (permute(Ubm[:, c0 * Dx:c1 * Dx, :], (0, 2, 1)) if kinner
+               else Ubm[:, c0 * Dx:c1 * Dx, :].transpose(1, 2))
+        gemm(UbT, Z2p.view(B, Dy * chi_a * Dy, nc * Dx).transpose(1, 2),
              outm.transpose(1, 2), beta=0.0 if c0 == 0 else 1.0, compute=compute)
         del Z1p, Z2p
     return permute(out, (0, 2, 4, 1, 3))                                   # [i,j,k,l]
