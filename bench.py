@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--atrg-q", type=int, default=2)
     ap.add_argument("--squeezer", default="QR", choices=["original", "QR", "fix"], help="hotrg3d_qr squeezer")
     ap.add_argument("--atrg-prec", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--merge3d", default="bf16x6", choices=["f32", "bf16x6"],
+                    help="hotrg3d: compute type of the steady steps' GEMMs (bf16x6 = three-piece split-bf16, "
+                         "fp32 accuracy)")
     ap.add_argument("--split6-env-from", type=int, default=6,
                     help="three-piece split-bf16 environments (fp32 accuracy) after this step (-1: never)")
     ap.add_argument("--split6-merge-from", type=int, default=6,
@@ -120,7 +123,8 @@ def _roof(rec, names, peak):
 def bench_hotrg3d(args):
     """Batched 3D Ising HOTRG: beta drawn from numpy.random.default_rng(0).uniform(0.20, 0.24, 128)
     (the first --nbeta of them), chi = 16, steps Z, Y, X (trg/HOTRG_3d.py:259-316) from the initial
-    tensor; float64 for the first --fp64-steps steps, then float32 MFMA with float64 eigensolvers."""
+    tensor; float64 for the first --fp64-steps steps, then three-piece split-bf16 GEMMs (fp32 accuracy;
+    --merge3d f32 for the float32 MFMA) with float64 eigensolvers."""
     import numpy as np
     import torch
 
@@ -129,7 +133,9 @@ def bench_hotrg3d(args):
     dev = torch.device("cuda:0")
     chi, nb, K = args.chi, args.nbeta, args.steps
     betas = np.random.default_rng(0).uniform(0.20, 0.24, 128)[:nb].tolist()
-    policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32)
+    split6 = args.fp64_steps if args.merge3d == "bf16x6" else None
+    policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32, split6_merge_from=split6,
+                          split6_env_from=split6)
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     T0 = torch.stack([models.initial_tensor("ising3d", b, dev) for b in betas])
     sched = [(n, "ZYX"[(n - 1) % 3]) for n in range(1, K + 1)]
@@ -204,7 +210,12 @@ def bench_hotrg3d(args):
     cfg = {"workload": f"3D Ising HOTRG chi={chi}, {K} RG steps (Z,Y,X) from the initial tensor, {nb} beta "
                        "from default_rng(0).uniform(0.20,0.24) batched",
            "chi": chi, "nbeta": nb, "rg_steps": K, "policy": policy.describe(), "parallelism": "single GPU, batched"}
-    roof = _roof(rec, "merge chi^11 GEMM (T.U1.U2 x T, f32 MFMA)", PEAK_TFLOPS["f32"])
+    if args.merge3d == "bf16x6":
+        roof = _roof(rec, "merge chi^11 GEMM (T.U1.U2 x T, bf16x6: six bf16 MFMAs per product)", PEAK_TFLOPS["bf16x6"])
+        if roof:
+            roof["mfma_issue_frac"] = roof["frac"] * MFMA_PER_PRODUCT["bf16x6"]
+    else:
+        roof = _roof(rec, "merge chi^11 GEMM (T.U1.U2 x T, f32 MFMA)", PEAK_TFLOPS["f32"])
     print(json.dumps(_line(cfg, nb * K / dt, K, args.warmup, dt, "f32", roof, extra)), flush=True)
     return 0
 
@@ -223,7 +234,9 @@ def bench_hotrg3d_qr(args):
     dev = torch.device("cuda:0")
     chi, nb, K = args.chi, args.nbeta, args.steps
     betas = np.random.default_rng(0).uniform(0.20, 0.24, 128)[:nb].tolist()
-    policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32)
+    split6 = args.fp64_steps if args.merge3d == "bf16x6" else None
+    policy = hotrg.Policy(fp64_steps=args.fp64_steps, low=hotrg.FP32, split6_merge_from=split6,
+                          split6_env_from=split6)
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     T0 = torch.stack([models.initial_tensor("ising3d", b, dev) for b in betas])
     order = hotrg_qr.schedule({"X": K, "Y": K, "Z": K})[:K]

@@ -219,3 +219,26 @@ def test_hotrg3d_nearest_two_point_matches_oracle(gpu):
     val = lambda r: np.exp(np.sum(r[3])) * np.einsum("xxyytt", r[1]) / np.einsum("xxyytt", r[0])
     assert np.allclose(got[2], ref[2], rtol=1e-11, atol=1e-14)
     assert abs(val(got) - val(ref)) < 1e-10 * abs(val(ref))
+
+
+def test_hotrg3d_split6_steps_match_fp32(gpu):
+    """The 3D step with three-piece split-bf16 GEMMs (fp32 accuracy; the chi^11
+    GEMM then reads T2 K-contiguous): ln Z/V after float64 steps 1-3 and three
+    split steps agrees with the float32 schedule and with float64 to 4e-8
+    (measured 2e-8 here, about twice the float32 schedule's error at this
+    early switch; two-piece split or plain bf16 would be off by 1e-7 and
+    more), and a batch gives each instance its single-instance bits."""
+    from trg_backup_amd import hotrg
+    betas = np.random.default_rng(0).uniform(0.20, 0.24, 128)[:3]
+    T0 = torch.stack([torch.as_tensor(models.ising3d_tensor(b)) for b in betas]).to(gpu)
+    p32 = hotrg.Policy(fp64_steps=3, low=hotrg.FP32)
+    p6 = hotrg.Policy(fp64_steps=3, low=hotrg.FP32, split6_merge_from=3, split6_env_from=3)
+    assert p6(4).merge == "bf16x6" and p6(4).env == "bf16x6"
+    z32 = hotrg.lnz3d(*hotrg.renorm3d(T0, 8, 2, 2, 2, p32), 6).cpu().numpy()
+    Tb, lb = hotrg.renorm3d(T0, 8, 2, 2, 2, p6)
+    z6 = hotrg.lnz3d(Tb, lb, 6).cpu().numpy()
+    zd = hotrg.lnz3d(*hotrg.renorm3d(T0, 8, 2, 2, 2, hotrg.ALL_FP64), 6).cpu().numpy()
+    print(f"\n[hotrg3d bf16x6] vs f32 {np.abs(z6 - z32)} vs f64 {np.abs(z6 - zd)}", flush=True)
+    assert np.all(np.abs(z6 - z32) < 4e-8) and np.all(np.abs(z6 - zd) < 4e-8)
+    Ti, li = hotrg.renorm3d(T0[1:2], 8, 2, 2, 2, p6)
+    assert hotrg.lnz3d(Ti, li, 6).cpu().numpy()[0] == z6[1]

@@ -393,8 +393,13 @@ def merge3d(T1, T2, U1a, U1b, U2a, U2b, compute=None):
             ops.copy_into(res[b0:b0 + outs[i].shape[0]], outs[i])
         return res
     out = torch.empty((B, ca, cb, cc_, cd, Dz, Dz), dtype=T1.dtype, device=T1.device)
-    # T2 as [(z,m,o),(n,p,f)]: permute legs (m,n,o,p,z,f) -> (z,m,o,n,p,f)
-    T2p = permute(T2, (0, 5, 1, 3, 2, 4, 6)).view(B, Dz * Dx * Dy, Dx * Dy * Dz)
+    # T2 as [(z,m,o),(n,p,f)]: permute legs (m,n,o,p,z,f) -> (z,m,o,n,p,f).  The bf16 MFMA
+    # wants K-contiguous operands: there T2 is stored [(n,p,f),(z,m,o)] and read transposed
+    # (same product bits in either layout)
+    if compute in SPLIT_COMPUTE:
+        T2p = permute(T2, (0, 2, 4, 6, 5, 1, 3)).view(B, Dx * Dy * Dz, Dz * Dx * Dy).transpose(1, 2)
+    else:
+        T2p = permute(T2, (0, 5, 1, 3, 2, 4, 6)).view(B, Dz * Dx * Dy, Dx * Dy * Dz)
     T1m = T1.reshape(B, Dx, Dx * Dy * Dy * Dz * Dz)                     # [i,(j,k,l,e,z)]
     for a0 in range(0, ca, na):
         a1 = min(ca, a0 + na)
