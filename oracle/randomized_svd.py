@@ -66,8 +66,10 @@ def rsvd2(A, k, n_oversamples=0, seed=1, del_A=False):
 def rsvd_for_3dATRG_tensor_init(A, B, Dcut, n_oversamples=10, n_power_iter=0, iterator="QR", seed=1234):
     """utility/randomized_svd.py:100-162: rSVD of the product A_{abci} B_{idef}.
 
-    parity unpinned beyond the dense-SVD equality test (the reference's
-    cuRAND draw cannot be reproduced).
+    Pinned by tests/test_oracle.py: with a sketch that covers the rank the
+    result equals the dense truncated SVD of the operator built from the
+    reference's own index strings (:119, :125), for both iterators; only the
+    reference's cuRAND draw cannot be reproduced (gauge-free checks).
     """
     l = Dcut + n_oversamples
     d = A.shape[:3]
@@ -99,7 +101,10 @@ def rsvd_for_3dATRG_tensor_inte(B, C, Dcut, n_oversamples=10, n_power_iter=0, it
     The reference contracts "iabc,ajkd,dbcl->ijkl" for the first sketch
     (:209) and "idbc,djka,abcl->ijkl" inside the iteration (:202); both are
     the same operator M with the row index (i,j,k) and the column index
-    (a,b,c).  parity unpinned beyond the dense-SVD equality test.
+    (a,b,c).  Pinned by tests/test_oracle.py: the four index strings of the
+    reference are checked: :202, :209 and :222 give M, :196 gives M^dag; and
+    with a sketch that covers the rank the result equals the dense
+    truncated SVD of M, for both iterators.
     """
     l = Dcut + n_oversamples
     Mop = np.einsum("idbc,djka->ijkabc", B, C).reshape(

@@ -180,6 +180,76 @@ def test_rsvd_exact_when_sketch_covers_rank():
     assert np.allclose((u * s) @ vh, A)
 
 
+def _check_truncated_factorisation(u, s, vh, M, k):
+    """u s vh must be the rank-k truncated SVD of M (gauge-free checks)."""
+    U0, s0, V0h = np.linalg.svd(M)
+    assert np.allclose(s, s0[:k], rtol=1e-12, atol=1e-12 * s0[0])
+    um = u.reshape(-1, u.shape[-1])
+    vm = vh.reshape(vh.shape[0], -1)
+    assert np.allclose(um.T @ um, np.eye(k), atol=1e-12)
+    assert np.allclose(um @ um.T, U0[:, :k] @ U0[:, :k].T, atol=1e-10)
+    assert np.allclose((um * s) @ vm, (U0[:, :k] * s0[:k]) @ V0h[:k], atol=1e-11 * s0[0])
+
+
+@pytest.mark.parametrize("iterator,q", [("QR", 0), ("QR", 2), ("power", 2)])
+def test_rsvd_for_3datrg_tensor_init_matches_dense_svd(iterator, q):
+    """utility/randomized_svd.py:100-162: the operator is "abci,idef->abcdef" (the sketch
+    product at :125 applied to the identity); its adjoint is the :119 string.  The inner
+    bond r=5 bounds the rank, and l = Dcut + p = 7 covers it, so the range finder is exact
+    and the result must be the dense truncated SVD."""
+    rng = np.random.default_rng(11)
+    D, r = 3, 5
+    A = rng.standard_normal((D, D, D, r))
+    B = rng.standard_normal((r, D, D, D))
+    M = np.einsum("abci,idef->abcdef", A, B).reshape(D ** 3, D ** 3)
+    Q = rng.standard_normal((D ** 3, 4))
+    adj = np.einsum("iabc,defi,defl->abcl", B, A, Q.reshape(D, D, D, 4)).reshape(D ** 3, 4)
+    assert np.allclose(adj, M.T @ Q)
+    u, s, vh = randomized_svd.rsvd_for_3dATRG_tensor_init(A, B, D, n_oversamples=4, n_power_iter=q,
+                                                           iterator=iterator, seed=5)
+    assert u.shape == (D, D, D, D) and s.shape == (D,) and vh.shape == (D, D, D, D)
+    _check_truncated_factorisation(u, s, vh, M, D)
+
+
+@pytest.mark.parametrize("iterator,q", [("QR", 0), ("QR", 2), ("power", 2)])
+def test_rsvd_for_3datrg_tensor_inte_matches_dense_svd(iterator, q):
+    """utility/randomized_svd.py:165-226: M_{(i,j,k),(a,b,c)} from the MQ string
+    "idbc,djka,abcl->ijkl" (:202) applied to the identity; the first sketch's string
+    "iabc,ajkd,dbcl->ijkl" (:209) and the Q^dag M string "abcl,adjk,dbci->lijk" (:222,
+    with Q the identity) must give the same operator, and "adjk,dbci,abcl->ijkl" (:196)
+    its adjoint.  With an
+    inner bond of 1 the rank is at most Dcut^2 = 9 and l = Dcut + p = 10 covers it."""
+    rng = np.random.default_rng(12)
+    D, x = 3, 1
+    B = rng.standard_normal((D, x, D, D))
+    C = rng.standard_normal((x, D, D, D))
+    n = D ** 3
+    eye = np.eye(n).reshape(D, D, D, n)
+    M = np.einsum("idbc,djka,abcl->ijkl", B, C, eye).reshape(n, n)
+    assert np.allclose(np.einsum("iabc,ajkd,dbcl->ijkl", B, C, eye).reshape(n, n), M)
+    assert np.allclose(np.einsum("abcl,adjk,dbci->lijk", eye, B, C).reshape(n, n), M)
+    assert np.allclose(np.einsum("adjk,dbci,abcl->ijkl", B, C, eye).reshape(n, n), M.T)
+    assert np.linalg.matrix_rank(M) <= D * D
+    u, s, vh = randomized_svd.rsvd_for_3dATRG_tensor_inte(B, C, D, n_oversamples=7, n_power_iter=q,
+                                                           iterator=iterator, seed=6)
+    assert u.shape == (D, D, D, D) and s.shape == (D,) and vh.shape == (D, D, D, D)
+    _check_truncated_factorisation(u, s, vh, M, D)
+
+
+def test_rsvd_for_3datrg_wide_sketch_ignores_the_draw():
+    """Two different seeds give the same truncated factorisation once the sketch covers
+    the rank (only the gauge of u and vh may differ), so a swapped or transposed operand
+    cannot hide behind the random draw."""
+    rng = np.random.default_rng(13)
+    A = rng.standard_normal((2, 2, 2, 3))
+    B = rng.standard_normal((3, 2, 2, 2))
+    a = randomized_svd.rsvd_for_3dATRG_tensor_init(A, B, 2, n_oversamples=2, seed=1)
+    b = randomized_svd.rsvd_for_3dATRG_tensor_init(A, B, 2, n_oversamples=2, seed=2)
+    assert np.allclose(a[1], b[1])
+    ua, ub = a[0].reshape(8, 2), b[0].reshape(8, 2)
+    assert np.allclose(ua @ ua.T, ub @ ub.T, atol=1e-10)
+
+
 def test_leg_swap_rsvd_matches_dense_svd():
     rng = np.random.default_rng(4)
     B = rng.standard_normal((4, 3, 5)); C = rng.standard_normal((5, 4, 3))
