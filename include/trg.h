@@ -175,6 +175,21 @@ int trg_eigh_topk_f64(int batch, int n, int k, int l, int q, const double *A, lo
                       void *ws, long long ws_bytes, int *info, void *stream);
 
 /*
+ * Allocation-free form of trg_eigh_topk_f64 (same kernels, same results bit
+ * for bit).  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The workspace is a file-scope
+ * device array of trg_eigh_topk_noalloc_max_bytes() bytes (64 MiB) that each
+ * GPU holds until the process ends; a call whose trg_eigh_topk_workspace
+ * exceeds it returns TRG_OOM.  All calls on one GPU share that array, so calls
+ * that overlap on one GPU (different streams or threads) race.
+ */
+long long trg_eigh_topk_noalloc_max_bytes(void);
+int trg_eigh_topk_f64_noalloc(int batch, int n, int k, int l, int q, const double *A, long long sA,
+                              double *w, double *V, double *eps, unsigned long long seed, int *info,
+                              void *stream);
+
+/*
  * HOTRG isometry choice on the device: for each b pick side 0 if
  * eps0[b] < eps1[b] else side 1 (trg/HOTRG_2d.py:45, trg/HOTRG_3d.py:76,83),
  * copy the chosen n x k block into U (stride n*k) and write the flag

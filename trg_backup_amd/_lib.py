@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtrg.so")
 
 F32, F64, BF16, I32, BF16X3, BF16X6 = 0, 1, 2, 3, 4, 5
-ABI = 5          # must match the "abi N" revision in trg_version()
+ABI = 6          # must match the "abi N" revision in trg_version()
 
 c_int, c_ll, c_dbl, c_vp, c_ull, c_uint = (ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p,
                                            ctypes.c_ulonglong, ctypes.c_uint)
@@ -42,6 +42,8 @@ SIGNATURES = {
     "trg_eigh_topk_workspace": [c_int, c_int, c_int],
     "trg_eigh_topk_f64": [c_int, c_int, c_int, c_int, c_int, c_vp, c_ll, c_vp, c_vp, c_vp, c_ull,
                           c_vp, c_ll, c_vp, c_vp],
+    "trg_eigh_topk_noalloc_max_bytes": [],
+    "trg_eigh_topk_f64_noalloc": [c_int, c_int, c_int, c_int, c_int, c_vp, c_ll, c_vp, c_vp, c_vp, c_ull, c_vp, c_vp],
     "trg_select_isometry": [c_int, c_int, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "trg_scale_rows": [c_int, c_int, c_ll, c_ll, c_ll, c_vp, c_ll, c_vp, c_ll, c_int, c_vp],
     "trg_lnz": [c_int, c_int, c_vp, c_ll, c_vp, c_dbl, c_vp, c_vp],
@@ -54,7 +56,7 @@ SIGNATURES = {
 }
 _RESTYPES = {"trg_version": ctypes.c_char_p, "trg_status_string": ctypes.c_char_p,
              "trg_gemm_config_name": ctypes.c_char_p,
-             "trg_eigh_topk_workspace": c_ll, "trg_orthonormalize_workspace": c_ll}
+             "trg_eigh_topk_workspace": c_ll, "trg_eigh_topk_noalloc_max_bytes": c_ll, "trg_orthonormalize_workspace": c_ll}
 
 
 class TrgError(RuntimeError):

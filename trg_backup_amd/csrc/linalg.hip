@@ -504,6 +504,53 @@ extern "C" int trg_eigh_topk_f64(int batch, int n, int k, int l, int q, const do
 }
 
 // ---------------------------------------------------------------------------
+// Allocation-free top-k eigensolver.  The workspace of trg_eigh_topk_f64
+// lives in a file-scope device array (each GPU holds its own copy, at its
+// own address, for the life of the process).  The array's name is a host
+// stand-in, not a device address: host code only ever passes the pointer
+// hipGetSymbolAddress returned for the current device.  Every workspace
+// word is written before it is read (Gaussian start block, Grams, Cholesky
+// factors, per-instance flags), so nothing needs clearing between calls.
+// Calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr long long EIGH_SCRATCH_BYTES = 64LL << 20;
+static __device__ __align__(256) unsigned char scratch[EIGH_SCRATCH_BYTES];
+
+static void *g_scratch_ptr[64];
+static int g_scratch_status[64];
+static std::once_flag g_scratch_once[64];
+
+// Device address of `scratch` on the current device (nullptr on failure).
+static void *scratch_ptr() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::call_once(g_scratch_once[dev], [dev]() {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_status[dev] = (int)e;
+        g_scratch_ptr[dev] = (e == hipSuccess) ? p : nullptr;
+    });
+    if (!g_scratch_ptr[dev]) g_last_hip_error = g_scratch_status[dev];
+    return g_scratch_ptr[dev];
+}
+
+extern "C" long long trg_eigh_topk_noalloc_max_bytes(void) { return EIGH_SCRATCH_BYTES; }
+
+extern "C" int trg_eigh_topk_f64_noalloc(int batch, int n, int k, int l, int q, const double *A, long long sA,
+                                         double *w, double *V, double *eps, unsigned long long seed, int *info,
+                                         void *stream) {
+    clear_hip_error();
+    if (batch < 0 || n < 0 || k < 0 || l < 0 || q < 0) return TRG_BAD_ARG;
+    if (batch == 0 || n == 0 || k == 0) return TRG_OK;
+    if (k > l || l > n || l > 256) return TRG_SHAPE;
+    const long long need = trg_eigh_topk_workspace(batch, n, l);
+    if (need > EIGH_SCRATCH_BYTES) return TRG_OOM;
+    void *ws = scratch_ptr();
+    if (!ws) return TRG_HIP_ERROR;
+    return trg_eigh_topk_f64(batch, n, k, l, q, A, sA, w, V, eps, seed, ws, EIGH_SCRATCH_BYTES, info, stream);
+}
+
+// ---------------------------------------------------------------------------
 // One-sided (Hestenes) Jacobi SVD of an m x n matrix, n <= m <= 256 per
 // instance, one workgroup each.  Columns are orthogonalised pairwise in the
 // round-robin order; singular values are the final column norms (high
