@@ -266,6 +266,19 @@ int trg_axpy_dev_f64(int batch, long long n, double alpha_rel, const double *sca
 int trg_lnz(int batch, int nterms, const double *lnf, long long lnf_stride, const double *tr,
             double inv_volume, double *out, void *stream);
 
+/*
+ * Solve health word, written without a host sync (graph-capturable):
+ * health[0] |= 1 if any of the n_info flags is non-zero (eigensolver,
+ * Jacobi or Cholesky breakdown flags of the calls above) and |= 2 if any of
+ * the n_vals normalisation constants is NaN, infinite or zero.  The host
+ * reads the word once at the end of a solve.  Replaces the Python
+ * exceptions a failed cupy.linalg.eigh / svd raises inside
+ * trg/HOTRG_2d.py:33-34, trg/HOTRG_3d.py:55-58, trg/ATRG_3d_new.py:175-188,
+ * and the division by a zero norm in trg/HOTRG_2d.py:181-185.
+ */
+int trg_health_or(long long n_info, const int *info, long long n_vals, const double *vals, int *health,
+                  void *stream);
+
 #ifdef __cplusplus
 }
 #endif

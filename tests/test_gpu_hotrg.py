@@ -242,3 +242,58 @@ def test_hotrg3d_split6_steps_match_fp32(gpu):
     assert np.all(np.abs(z6 - z32) < 4e-8) and np.all(np.abs(z6 - zd) < 4e-8)
     Ti, li = hotrg.renorm3d(T0[1:2], 8, 2, 2, 2, p6)
     assert hotrg.lnz3d(Ti, li, 6).cpu().numpy()[0] == z6[1]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x6"])
+def test_merge3d_output_chunks_bitwise_equal_one_chunk(gpu, monkeypatch, prec):
+    """merge3d splits the output leg a into chunks (and the batch into sub-batches) once the
+    chi^8 intermediates exceed the budgets, as at chi = 16.  Chunking over an output leg
+    changes no element's reduction, so the result must be bit for bit the one-chunk result,
+    in every compute type."""
+    from trg_backup_amd import hotrg, ops
+    T = torch.stack([torch.as_tensor(models.ising3d_tensor(b)) for b in (0.215, 0.228)]).to(gpu)
+    T, _ = hotrg.renorm3d(T, 6, 1, 1, 1, hotrg.ALL_FP64)
+    U1, U2, _, _ = hotrg.projector3d(T, 6)
+    store = torch.float64 if prec == "f64" else torch.float32
+    compute = None if prec in ("f64", "f32") else prec
+    T, U1, U2 = ops.cast(T, store), ops.cast(U1, store), ops.cast(U2, store)
+    ref = hotrg.merge3d(T, T, U1, U1, U2, U2, compute=compute)
+    Dx, Dy, Dz = T.shape[1], T.shape[3], T.shape[5]
+    per_a = 2 * Dx * Dy * Dz * U2.shape[3] * Dx * Dy * Dz * T.element_size()
+    monkeypatch.setattr(hotrg, "INSTANCE_BUDGET", per_a * 4)     # a in chunks of 4 (6 = 4 + 2)
+    monkeypatch.setattr(hotrg, "MERGE_BUDGET", per_a * 4)        # one instance per sub-batch
+    out = hotrg.merge3d(T, T, U1, U1, U2, U2, compute=compute)
+    torch.cuda.synchronize()
+    print(f"\n[merge3d chunks {prec}] max |diff| {float((out.double() - ref.double()).abs().max()):.3e}", flush=True)
+    assert torch.equal(out, ref)
+
+
+def test_hotrg3d_chunked_merge_matches_oracle(gpu, monkeypatch):
+    """The same chunked merge inside a float64 solve (chi = 6, 9 steps) against the oracle."""
+    from trg_backup_amd import hotrg
+    monkeypatch.setattr(hotrg, "INSTANCE_BUDGET", 2 * 6 ** 7 * 8 * 2)
+    monkeypatch.setattr(hotrg, "MERGE_BUDGET", 2 * 6 ** 7 * 8 * 2)
+    betas = (0.215, 0.228)
+    T = torch.stack([torch.as_tensor(models.ising3d_tensor(b)) for b in betas]).to(gpu)
+    Td, ld = hotrg.renorm3d(T, 6, 3, 3, 3, hotrg.ALL_FP64)
+    v = hotrg.lnz3d(Td, ld, 9).cpu().numpy()
+    for i, b in enumerate(betas):
+        ref = o3.ln_Z_over_V(models.ising3d_tensor(b), 6, 3, 3, 3)
+        print(f"\n[hotrg3d chunked chi=6 beta={b}] gpu={v[i]!r} oracle={ref!r} diff={v[i] - ref:.3e}", flush=True)
+        assert abs(v[i] - ref) <= 1e-11 * abs(ref)
+
+
+def test_merge2d_chunks_match_one_chunk_and_oracle(gpu, monkeypatch):
+    """merge2d sums the contracted c leg over chunks (the last GEMM accumulates with beta = 1):
+    a chunked chi = 16 solve matches the oracle to 1e-12 and the one-chunk solve to rounding."""
+    from trg_backup_amd import hotrg
+    T = torch.as_tensor(models.ising2d_tensor(0.44)).to(gpu)[None]
+    Ta, la = hotrg.renorm2d(T, 16, 8, 8, hotrg.ALL_FP64)
+    a = float(hotrg.lnz2d(Ta, la, 16)[0])
+    monkeypatch.setattr(hotrg, "INSTANCE_BUDGET", 2 * 16 ** 4 * 8 * 3)   # 3 c per chunk: 16 = 5 x 3 + 1
+    Tb, lb = hotrg.renorm2d(T, 16, 8, 8, hotrg.ALL_FP64)
+    b = float(hotrg.lnz2d(Tb, lb, 16)[0])
+    ref = o2.ln_Z_over_V(models.ising2d_tensor(0.44), 16, 8, 8)
+    print(f"\n[merge2d chunks] one chunk {a!r} chunked {b!r} oracle {ref!r}", flush=True)
+    assert abs(b - ref) <= 1e-12 * abs(ref)
+    assert abs(b - a) <= 1e-13 * abs(a)

@@ -63,3 +63,46 @@ def test_precision_policy_tiers():
     assert (pol(23).env, pol(23).merge) == ("bf16", "bf16")
     assert pol(7).env_compute() == "bf16x6" and pol(7).store == torch.float32
     assert "bf16x6 env after step 6" in pol.describe()
+
+
+class _CpuOps:
+    """The two copies merge3d makes through libtrg, emulated on the CPU."""
+
+    @staticmethod
+    def contiguous(t, dtype=None):
+        return t.contiguous().to(dtype or t.dtype)
+
+    @staticmethod
+    def copy_into(dst, src):
+        dst.copy_(src)
+        return dst
+
+
+@pytest.mark.parametrize("shape", [(2, 3, 2, 2, 5, 4, 3, 2), (3, 2, 3, 2, 3, 3, 4, 4), (1, 4, 4, 2, 6, 6, 5, 5)])
+@pytest.mark.parametrize("compute", [None, "bf16x6"])
+@pytest.mark.parametrize("chunks", ["one", "a-chunks", "a-chunks+sub-batches"])
+def test_merge3d_chain_layouts(monkeypatch, shape, compute, chunks):
+    """ijklez,mnopzf,ima,jnb,koc,lpd->abcdef (trg/HOTRG_3d.py:107-116) through merge3d's
+    five-GEMM chain, with the output leg a in one chunk, in chunks of two (the last one
+    shorter when ca is odd), and additionally split into sub-batches of one instance;
+    compute bf16x6 takes the K-contiguous layout of T2."""
+    B, Dx, Dy, Dz, ca, cb, cc, cd = shape
+    monkeypatch.setattr(hotrg, "gemm", _cpu_gemm)
+    monkeypatch.setattr(hotrg, "permute", lambda t, perm, dtype=None: t.permute(*perm).contiguous().to(dtype or t.dtype))
+    monkeypatch.setattr(hotrg, "ops", _CpuOps)
+    per_a = 2 * Dx * Dy * Dz * cc * Dx * Dy * Dz * 8
+    if chunks != "one":
+        monkeypatch.setattr(hotrg, "INSTANCE_BUDGET", per_a * 2)
+    if chunks == "a-chunks+sub-batches":
+        monkeypatch.setattr(hotrg, "MERGE_BUDGET", per_a * 2)
+    g = torch.Generator().manual_seed(sum(shape))
+    T1 = torch.randn(B, Dx, Dx, Dy, Dy, Dz, Dz, dtype=torch.float64, generator=g)
+    T2 = torch.randn(B, Dx, Dx, Dy, Dy, Dz, Dz, dtype=torch.float64, generator=g)
+    U1a = torch.randn(B, Dx, Dx, ca, dtype=torch.float64, generator=g)
+    U1b = torch.randn(B, Dx, Dx, cb, dtype=torch.float64, generator=g)
+    U2a = torch.randn(B, Dy, Dy, cc, dtype=torch.float64, generator=g)
+    U2b = torch.randn(B, Dy, Dy, cd, dtype=torch.float64, generator=g)
+    ref = torch.einsum("Zijklez,Zmnopzf,Zima,Zjnb,Zkoc,Zlpd->Zabcdef", T1, T2, U1a, U1b, U2a, U2b)
+    out = hotrg.merge3d(T1, T2, U1a, U1b, U2a, U2b, compute=compute)
+    assert out.shape == ref.shape
+    assert (out - ref).abs().max().item() < 1e-10 * ref.abs().max().item()

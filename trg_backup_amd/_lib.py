@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtrg.so")
 
 F32, F64, BF16, I32, BF16X3, BF16X6 = 0, 1, 2, 3, 4, 5
-ABI = 6          # must match the "abi N" revision in trg_version()
+ABI = 7          # must match the "abi N" revision in trg_version()
 
 c_int, c_ll, c_dbl, c_vp, c_ull, c_uint = (ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p,
                                            ctypes.c_ulonglong, ctypes.c_uint)
@@ -53,6 +53,7 @@ SIGNATURES = {
     "trg_orthonormalize_workspace": [c_int, c_int, c_int],
     "trg_orthonormalize_f64": [c_int, c_int, c_int, c_vp, c_ll, c_vp, c_vp, c_ll, c_int, c_vp, c_vp],
     "trg_axpy_dev_f64": [c_int, c_ll, c_dbl, c_vp, c_vp, c_ll, c_vp, c_ll, c_vp],
+    "trg_health_or": [c_ll, c_vp, c_ll, c_vp, c_vp, c_vp],
 }
 _RESTYPES = {"trg_version": ctypes.c_char_p, "trg_status_string": ctypes.c_char_p,
              "trg_gemm_config_name": ctypes.c_char_p,

@@ -299,6 +299,26 @@ __global__ void gaussian_kernel(long long n, unsigned long long seed, unsigned i
 }
 
 // lnz[b] = sum_i lnf[b, i] + ln|tr_b| * inv_volume (fixed summation order)
+// health[0] |= 1 if any info[i] != 0, |= 2 if any vals[i] is NaN, infinite or zero
+// (one workgroup; a wavefront OR-reduction, then one lane per wave into LDS).
+__global__ void __launch_bounds__(256) health_or_kernel(long long n_info, const int *info, long long n_vals,
+                                                        const double *vals, int *health) {
+    __shared__ int part[4];
+    int bits = 0;
+    for (long long i = threadIdx.x; i < n_info; i += blockDim.x) bits |= info[i] != 0 ? 1 : 0;
+    for (long long i = threadIdx.x; i < n_vals; i += blockDim.x) {
+        const double v = vals[i];
+        bits |= (v != v || fabs(v) > 1.7976931348623157e308 || v == 0.0) ? 2 : 0;
+    }
+    for (int off = 32; off > 0; off >>= 1) bits |= __shfl_xor(bits, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = bits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int all = part[0] | part[1] | part[2] | part[3];
+        if (all) health[0] = health[0] | all;
+    }
+}
+
 __global__ void lnz_kernel(int batch, int nterms, const double *lnf, long long lnf_stride, const double *tr,
                            double inv_volume, double *out) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -551,5 +571,15 @@ extern "C" int trg_lnz(int batch, int nterms, const double *lnf, long long lnf_s
     if (!lnf || !out) return TRG_BAD_ARG;
     hipLaunchKernelGGL(lnz_kernel, (unsigned)cdiv(batch, 256), 256, 0, as_stream(stream), batch, nterms, lnf,
                        lnf_stride, tr, inv_volume, out);
+    return check_launch();
+}
+
+extern "C" int trg_health_or(long long n_info, const int *info, long long n_vals, const double *vals, int *health,
+                             void *stream) {
+    clear_hip_error();
+    if (n_info < 0 || n_vals < 0) return TRG_BAD_ARG;
+    if (!health || (n_info > 0 && !info) || (n_vals > 0 && !vals)) return TRG_BAD_ARG;
+    if (n_info == 0 && n_vals == 0) return TRG_OK;
+    hipLaunchKernelGGL(health_or_kernel, 1, 256, 0, as_stream(stream), n_info, info, n_vals, vals, health);
     return check_launch();
 }

@@ -32,25 +32,33 @@ def set_step(counter, n):
 
 
 class GraphedStep2D:
-    def __init__(self, batch, chi, direction, prec, lnf, step, warmup=1):
+    """`health` (an ops.Health, optional) receives the breakdown flags and bad
+    norms of every replay; check() reads it."""
+
+    def __init__(self, batch, chi, direction, prec, lnf, step, warmup=1, health=None):
         dev = lnf.device
         self.batch, self.chi, self.direction, self.prec = batch, chi, direction, prec
         self.inp = torch.empty((batch,) + (chi,) * getattr(self, "ndim", 4), dtype=prec.store, device=dev)
         ops.fill_(self.inp, 1.0)
-        # warm-up on a side stream writes into scratch log-norm / counter buffers
+        self.health = health if health is not None else ops.Health(dev)
+        # warm-up on a side stream writes into scratch log-norm / counter buffers (its dummy
+        # input reports to no health word)
         scratch_lnf = ops.fill_(torch.empty(lnf.shape, dtype=torch.float64, device=dev), 0.0)
         scratch_step = step_counter(dev)
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
+        with torch.cuda.stream(s), ops.isolated_health(None):
             for _ in range(warmup):
                 self._body(scratch_lnf, scratch_step)
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
+        with torch.cuda.graph(self.graph), ops.isolated_health(self.health):
             self.out = self._body(lnf, step)
         torch.cuda.synchronize()
+
+    def check(self):
+        self.health.check(f"graphed {self.direction} step")
 
     def _body(self, lnf, step):
         T = hotrg.step2d(self.inp, self.chi, self.direction, self.prec)
@@ -68,9 +76,9 @@ class GraphedStep2D:
 class GraphedStep3D(GraphedStep2D):
     """A captured steady-state batched 3D HOTRG step (legs chi^6)."""
 
-    def __init__(self, batch, chi, direction, prec, lnf, step, warmup=1):
+    def __init__(self, batch, chi, direction, prec, lnf, step, warmup=1, health=None):
         self.ndim = 6
-        super().__init__(batch, chi, direction, prec, lnf, step, warmup)
+        super().__init__(batch, chi, direction, prec, lnf, step, warmup, health)
 
     def _body(self, lnf, step):
         T = hotrg.step3d(self.inp, self.chi, self.direction, self.prec)

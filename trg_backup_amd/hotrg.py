@@ -291,22 +291,25 @@ def renorm2d(T, Dcut, XLOOPS, YLOOPS, policy=ALL_FP64, sync_each_step=False):
     B = T.shape[0]
     lnf = torch.empty((B, XLOOPS + YLOOPS + 1), dtype=torch.float64, device=T.device)
     ops.fill_(lnf, 0.0)
-    T = ops.contiguous(T, policy(0).store) if T.dtype != policy(0).store else ops.contiguous(T)
-    normalize_maxabs(T, lnf, 0)
-    nx = ny = n = 0
-    while nx < XLOOPS or ny < YLOOPS:
-        for direction in ("Y", "X"):
-            if direction == "Y" and ny < YLOOPS:
-                ny += 1
-            elif direction == "X" and nx < XLOOPS:
-                nx += 1
-            else:
-                continue
-            n += 1
-            T = step2d(T, Dcut, direction, policy(n))
-            normalize_maxabs(T, lnf, n)
-            if sync_each_step:
-                torch.cuda.current_stream().synchronize()
+    health = ops.Health(T.device)
+    with health:
+        T = ops.contiguous(T, policy(0).store) if T.dtype != policy(0).store else ops.contiguous(T)
+        normalize_maxabs(T, lnf, 0)
+        nx = ny = n = 0
+        while nx < XLOOPS or ny < YLOOPS:
+            for direction in ("Y", "X"):
+                if direction == "Y" and ny < YLOOPS:
+                    ny += 1
+                elif direction == "X" and nx < XLOOPS:
+                    nx += 1
+                else:
+                    continue
+                n += 1
+                T = step2d(T, Dcut, direction, policy(n))
+                normalize_maxabs(T, lnf, n)
+                if sync_each_step:
+                    torch.cuda.current_stream().synchronize()
+    health.check("HOTRG 2D solve")
     return T, lnf
 
 
@@ -473,20 +476,23 @@ def renorm3d(T, Dcut, XLOOPS, YLOOPS, ZLOOPS, policy=ALL_FP64, sync_each_step=Fa
     B = T.shape[0]
     lnf = torch.empty((B, XLOOPS + YLOOPS + ZLOOPS + 1), dtype=torch.float64, device=T.device)
     ops.fill_(lnf, 0.0)
-    T = ops.contiguous(T, policy(0).store)
-    normalize_trace(T, lnf, 0)
-    cnt = {"X": 0, "Y": 0, "Z": 0}
-    tot = {"X": XLOOPS, "Y": YLOOPS, "Z": ZLOOPS}
-    n = 0
-    while any(cnt[d] < tot[d] for d in "XYZ"):
-        for direction in ("Z", "Y", "X"):
-            if cnt[direction] < tot[direction]:
-                cnt[direction] += 1
-                n += 1
-                T = step3d(T, Dcut, direction, policy(n))
-                normalize_trace(T, lnf, n)
-                if sync_each_step:
-                    torch.cuda.current_stream().synchronize()
+    health = ops.Health(T.device)
+    with health:
+        T = ops.contiguous(T, policy(0).store)
+        normalize_trace(T, lnf, 0)
+        cnt = {"X": 0, "Y": 0, "Z": 0}
+        tot = {"X": XLOOPS, "Y": YLOOPS, "Z": ZLOOPS}
+        n = 0
+        while any(cnt[d] < tot[d] for d in "XYZ"):
+            for direction in ("Z", "Y", "X"):
+                if cnt[direction] < tot[direction]:
+                    cnt[direction] += 1
+                    n += 1
+                    T = step3d(T, Dcut, direction, policy(n))
+                    normalize_trace(T, lnf, n)
+                    if sync_each_step:
+                        torch.cuda.current_stream().synchronize()
+    health.check("HOTRG 3D solve")
     return T, lnf
 
 

@@ -171,22 +171,25 @@ def renorm(T, Dcut, TOT_RGSTEPS, kind="QR", deg_eps=0.0, policy=hotrg.ALL_FP64, 
     the values the reference stores under the keys "(X,Y,Z)"."""
     order = schedule(TOT_RGSTEPS)
     B = T.shape[0]
-    T = ops.contiguous(T, policy(0).store)
-    lnf = torch.empty((B, len(order) + 1), dtype=torch.float64, device=T.device)
-    ops.fill_(lnf, 0.0)
-    hotrg.normalize_trace(T, lnf, 0)
-    groups = [(list(range(B)), T, lnf)]
-    for n, direction in enumerate(order, start=1):
-        nxt = []
-        for idx, Tg, lg in groups:
-            for sub, Tn in step(Tg, Dcut, direction, kind, deg_eps, policy(n)):
-                ln = _take(lg, sub)
-                Tn = ops.contiguous(Tn)
-                hotrg.normalize_trace(Tn, ln, n)
-                nxt.append(([idx[i] for i in sub], Tn, ln))
-        groups = nxt
-        if sync_each_step:
-            torch.cuda.current_stream().synchronize()
+    health = ops.Health(T.device)
+    with health:
+        T = ops.contiguous(T, policy(0).store)
+        lnf = torch.empty((B, len(order) + 1), dtype=torch.float64, device=T.device)
+        ops.fill_(lnf, 0.0)
+        hotrg.normalize_trace(T, lnf, 0)
+        groups = [(list(range(B)), T, lnf)]
+        for n, direction in enumerate(order, start=1):
+            nxt = []
+            for idx, Tg, lg in groups:
+                for sub, Tn in step(Tg, Dcut, direction, kind, deg_eps, policy(n)):
+                    ln = _take(lg, sub)
+                    Tn = ops.contiguous(Tn)
+                    hotrg.normalize_trace(Tn, ln, n)
+                    nxt.append(([idx[i] for i in sub], Tn, ln))
+            groups = nxt
+            if sync_each_step:
+                torch.cuda.current_stream().synchronize()
+    health.check("HOTRG_3d_QR solve")
     return groups
 
 

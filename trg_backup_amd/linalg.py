@@ -39,6 +39,7 @@ def svd_small(A, batch=None):
     V = torch.empty((B, n, n), dtype=torch.float64, device=A.device)
     info = torch.empty(B, dtype=torch.int32, device=A.device)
     call("trg_gesvj_f64", B, m, n, ptr(A), m * n, ptr(s), ptr(U), ptr(V), 60, ptr(info), stream_ptr())
+    ops.note(info=info)
     return U, s, V.transpose(1, 2)
 
 
@@ -68,7 +69,10 @@ def orthonormalize(X, perturb=PERTURB, stream_id=7, passes=3, return_info=False)
     call("trg_orthonormalize_f64", B, m, l, ptr(X), m * l, ptr(Q), ptr(ws), ws.numel() * 8, passes, ptr(info),
          stream_ptr())
     # info[b] != 0: a Cholesky pivot fell to the rounding level (the block is
-    # rank deficient beyond what the perturbation repairs)
+    # rank deficient beyond what the perturbation repairs).  A caller that asks
+    # for the flags handles them; otherwise they go to the solve's health word
+    if not return_info:
+        ops.note(info=info)
     return (Q, info) if return_info else Q
 
 

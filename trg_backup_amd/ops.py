@@ -3,7 +3,9 @@
 torch is used for device memory and the current stream only; every
 arithmetic operation below is a libtrg kernel.
 """
+import contextlib
 import math
+import threading
 
 import torch
 
@@ -96,6 +98,77 @@ def fill_(t, value):
     code = I32 if t.dtype == torch.int32 else dtype_code(t)
     call("trg_fill", code, t.numel(), float(value), ptr(t), stream_ptr())
     return t
+
+
+# ---------------------------------------------------------------------------
+# solve health: breakdown flags and bad normalisations, checked once per solve
+# ---------------------------------------------------------------------------
+_SCOPES = threading.local()
+
+
+class Health:
+    """A device-side failure word for one solve (or one captured graph).
+
+    While the scope is entered on a host thread, every eigensolver, Jacobi SVD
+    and orthonormalisation call on that thread ORs its breakdown flags into the
+    word (bit 1), and every normalisation ORs in a NaN, infinite or zero norm
+    (bit 2) -- all on the device, without a host sync, so a captured graph
+    keeps doing it on every replay.  check() synchronises the stream once and
+    raises TrgError if any bit is set."""
+
+    BITS = {1: "an eigensolver / SVD / Cholesky breakdown flag", 2: "a NaN, infinite or zero normalisation constant"}
+
+    def __init__(self, device, word=None):
+        self.word = word if word is not None else fill_(torch.empty(1, dtype=torch.int32, device=device), 0)
+
+    def __enter__(self):
+        stack = getattr(_SCOPES, "stack", None)
+        if stack is None:
+            stack = _SCOPES.stack = []
+        stack.append(self)
+        return self
+
+    def __exit__(self, *exc):
+        _SCOPES.stack.remove(self)
+        return False
+
+    def value(self):
+        return int(self.word.cpu().item())
+
+    def check(self, what="solve"):
+        v = self.value()
+        if v:
+            why = "; ".join(m for b, m in self.BITS.items() if v & b)
+            raise _lib.TrgError(f"{what} failed on the device: {why}")
+
+
+@contextlib.contextmanager
+def isolated_health(h=None):
+    """Run a block in which only `h` (or no scope at all) receives health notes
+    on this thread, e.g. the warm-up and capture of a HIP graph."""
+    old = getattr(_SCOPES, "stack", None)
+    _SCOPES.stack = [h] if h is not None else []
+    try:
+        yield h
+    finally:
+        _SCOPES.stack = old
+
+
+def note(info=None, vals=None):
+    """OR breakdown flags (int32 device tensor) and normalisation constants
+    (float64 device tensor) into every Health scope active on this thread."""
+    stack = getattr(_SCOPES, "stack", None)
+    if not stack:
+        return
+    ni = info.numel() if info is not None else 0
+    nv = vals.numel() if vals is not None else 0
+    if ni == 0 and nv == 0:
+        return
+    if info is not None and not info.is_contiguous():
+        info = info.contiguous()
+    for h in stack:
+        call("trg_health_or", ni, ptr(info) if ni else None, nv, ptr(vals) if nv else None, ptr(h.word),
+             stream_ptr())
 
 
 # ---------------------------------------------------------------------------
@@ -281,6 +354,7 @@ def normalize_(T, c, lnf=None, step=0, batch=1, lnf_stride=0, step_ptr=None):
     n = T.numel() // batch
     call("trg_normalize", dtype_code(T), batch, n, ptr(T), n, ptr(c), ptr(lnf) if lnf is not None else None,
          lnf_stride, int(step), ptr(step_ptr) if step_ptr is not None else None, stream_ptr())
+    note(vals=c if c.dtype == torch.float64 and c.is_contiguous() else None)
     return T
 
 
@@ -304,6 +378,7 @@ def syevj(A, batch=1, max_sweeps=40):
     V = torch.empty((batch, n, n), dtype=torch.float64, device=A.device)
     info = torch.empty(batch, dtype=torch.int32, device=A.device)
     call("trg_syevj_f64", batch, n, ptr(A), n * n, ptr(w), ptr(V), max_sweeps, ptr(info), stream_ptr())
+    note(info=info)
     return w, V, info
 
 
@@ -338,6 +413,7 @@ def eigh_topk(A, k, l, q, seed=20240, batch=1):
     info = torch.empty(batch, dtype=torch.int32, device=A.device)
     call("trg_eigh_topk_f64", batch, n, k, l, q, ptr(A), n * n, ptr(w), ptr(V), ptr(eps), int(seed), ptr(ws),
          ws.numel() * 8, ptr(info), stream_ptr())
+    note(info=info)
     return w, V, eps, info
 
 

@@ -64,6 +64,7 @@ class Flow:
     admitted_at: float = 0.0
     result: float = None
     finished_at: float = None
+    error: str = None          # set when a step's health word reports a breakdown or a bad norm
 
 
 def footprint(req, policy=None):
@@ -129,6 +130,23 @@ class Server:
         stream as soon as they are released."""
         keep = []
         with torch.cuda.stream(stream):
+            health = ops.Health(stream.device)
+            health.__enter__()
+        try:
+            self._step_group_on(flows, stream, keep)
+        finally:
+            health.__exit__(None, None, None)
+        stream.synchronize()
+        bad = health.value()
+        if bad:
+            msg = "; ".join(m for b, m in ops.Health.BITS.items() if bad & b)
+            for f in flows:
+                f.error = f"step {f.step} failed on the device: {msg}"
+        del keep
+        return flows
+
+    def _step_group_on(self, flows, stream, keep):
+        with torch.cuda.stream(stream):
             if flows[0].scheme == "hotrg2d":
                 n0 = flows[0].step + 1
                 prec = self.policy(n0)
@@ -158,13 +176,13 @@ class Server:
                     f.U, f.s, f.VH = U, s, VH
                     f.consts.append(c)
                     f.step = n
-            stream.synchronize()
-        del keep
-        return flows
 
     def _finish(self, fl, now):
         r = fl.req
-        if fl.scheme == "hotrg2d":
+        if fl.error:
+            fl.result = None
+            fl.T = fl.lnf = fl.U = fl.s = fl.VH = None
+        elif fl.scheme == "hotrg2d":
             fl.result = float(hotrg.lnz2d(fl.T, fl.lnf, r.steps)[0].item())
             fl.T = fl.lnf = None
         else:
@@ -251,13 +269,14 @@ class Server:
                 fu.result()
                 free.append(si)
                 for f in group:
-                    if f.step >= f.req.steps:
+                    if f.step >= f.req.steps or f.error:
                         self._finish(f, now())
                         used -= f.footprint
                         done.append(f)
                         if log:
+                            res = f"lnZ/V={f.result:.12f}" if f.error is None else f"error: {f.error}"
                             log(f"done rid={f.req.rid} {f.req.model} chi={f.req.chi} steps={f.req.steps} "
-                                f"lnZ/V={f.result:.12f} latency={f.finished_at - f.req.arrival:.3f}s")
+                                f"{res} latency={f.finished_at - f.req.arrival:.3f}s")
                     else:
                         ready.append(f)
         return done

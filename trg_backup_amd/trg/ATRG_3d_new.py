@@ -21,6 +21,7 @@ import torch
 
 from .. import atrg, linalg, ops
 from .._io import from_device, to_device
+from .._lib import TrgError
 
 atrg.degeneracy_eps = float(os.environ.get("DGENERACY_EPS", "0.0"))
 degeneracy_eps = atrg.degeneracy_eps
@@ -142,23 +143,35 @@ def _reset_steps():
     return rgstep
 
 
+def _log_norm(c, nsteps):
+    """ln|c| / 2^nsteps of a device normalisation constant; a zero or non-finite
+    norm (a tensor that vanished or overflowed) raises instead of returning inf."""
+    v = abs(float(c[0].item()))
+    if not (v > 0.0 and math.isfinite(v)):
+        raise TrgError(f"ATRG normalisation constant is {v} after {nsteps} RG steps")
+    return math.log(v) / 2 ** nsteps
+
+
 def pure_tensor(T, Dcut, TOT_RGSTEPS):
     """:974-1039: cycle T, X, Y until the totals are reached.  The tensor stays
     on the GPU between steps and is returned in the caller's array kind."""
     U, s, VH, kind = _dev3(T)
     rgstep = _reset_steps()
-    s, c = atrg.normalize(U, s, VH)
-    T.normal_const[f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"] = math.log(abs(float(c[0].item())))
-    total = sum(TOT_RGSTEPS.values())
-    for direction in itertools.cycle("TXY"):
-        if sum(rgstep.values()) >= total:
-            break
-        if rgstep[direction] < TOT_RGSTEPS[direction]:
-            rgstep[direction] += 1
-            U, s, VH = atrg.atrg_pure_tensor(U, s, VH, Dcut, direction, _step_precision(sum(rgstep.values())))
-            s, c = atrg.normalize(U, s, VH)
-            key = f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
-            T.normal_const[key] = math.log(abs(float(c[0].item()))) / 2 ** (sum(rgstep.values()))
+    health = ops.Health(U.device)
+    with health:
+        s, c = atrg.normalize(U, s, VH)
+        T.normal_const[f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"] = _log_norm(c, 0)
+        total = sum(TOT_RGSTEPS.values())
+        for direction in itertools.cycle("TXY"):
+            if sum(rgstep.values()) >= total:
+                break
+            if rgstep[direction] < TOT_RGSTEPS[direction]:
+                rgstep[direction] += 1
+                U, s, VH = atrg.atrg_pure_tensor(U, s, VH, Dcut, direction, _step_precision(sum(rgstep.values())))
+                s, c = atrg.normalize(U, s, VH)
+                key = f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
+                T.normal_const[key] = _log_norm(c, sum(rgstep.values()))
+    health.check("ATRG solve")
     T.update(from_device(U, kind), from_device(s, kind), from_device(VH, kind))
     return T
 
