@@ -22,10 +22,14 @@ def _reqs():
 def test_each_result_is_the_same_alone_and_in_the_mix(gpu):
     from trg_backup_amd.serve import Server
     reqs = _reqs()
-    mixed = {f.req.rid: f.result for f in Server(streams=2).run(reqs, realtime=False)}
+    done = Server(streams=2).run(reqs, realtime=False)
+    assert all(f.error is None and np.isfinite(f.result) for f in done), [(f.req.rid, f.error) for f in done]
+    mixed = {f.req.rid: f.result for f in done}
     alone = {}
     for r in reqs:
-        alone[r.rid] = Server(streams=1).run([r], realtime=False)[0].result
+        f = Server(streams=1).run([r], realtime=False)[0]
+        assert f.error is None, f.error
+        alone[r.rid] = f.result
     for rid in alone:
         print(f"\n[serve] rid={rid} alone={alone[rid]!r} mixed={mixed[rid]!r}", flush=True)
         assert mixed[rid] == alone[rid], (rid, mixed[rid], alone[rid])

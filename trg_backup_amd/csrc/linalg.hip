@@ -572,9 +572,22 @@ __global__ void __launch_bounds__(256) gesvj_kernel(int m, int n, const double *
     double *U = Uout + (long long)b * m * n;
     double *V = Vout + (long long)b * n * n;
     double *W = use_lds ? lds : U;     // work columns (row-major m x n)
-    for (int e = tid; e < m * n; e += 256) W[e] = A[e];
+    __shared__ double red[4];
+    double fro = 0.0;
+    for (int e = tid; e < m * n; e += 256) {
+        const double x = A[e];
+        W[e] = x;
+        fro += x * x;
+    }
     for (int e = tid; e < n * n; e += 256) V[e] = (e / n == e % n) ? 1.0 : 0.0;
+    for (int off = 32; off > 0; off >>= 1) fro += __shfl_xor(fro, off);
+    if (lane == 0) red[wave] = fro;
     __syncthreads();
+    // Pairs whose inner product sits below (eps ||A||_F)^2 are noise-level
+    // columns of a rank-deficient matrix: rotating them changes nothing
+    // above rounding, and a purely relative test would keep rotating them
+    // for ever (no convergence); they are left alone.
+    const double floor_g = 4.93e-32 * (red[0] + red[1] + red[2] + red[3]);
     const int N = n + (n & 1);
     const int npair = N / 2;
     int sweep = 0;
@@ -600,7 +613,7 @@ __global__ void __launch_bounds__(256) gesvj_kernel(int m, int n, const double *
                 }
                 if (lane == 0) {
                     double c = 1.0, s = 0.0;
-                    if (q < n && fabs(g) > 2.2e-16 * sqrt(a * bq) && g != 0.0) {
+                    if (q < n && fabs(g) > fmax(2.2e-16 * sqrt(a * bq), floor_g) && g != 0.0) {
                         const double zeta = (bq - a) / (2.0 * g);
                         const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                         c = 1.0 / sqrt(1.0 + t * t);
