@@ -144,18 +144,23 @@ def bench_hotrg3d(args):
     T0 = torch.stack([models.initial_tensor("ising3d", b, dev) for b in betas])
     sched = workloads.schedule3d(K)
 
-    def run(steps, pol, T0_):
+    def run(steps, pol, T0_, what=None):
         lnf = ops.fill_(torch.empty((T0_.shape[0], K + 1), dtype=torch.float64, device=dev), 0.0)
         T = ops.contiguous(T0_, pol(0).store)
         hotrg.normalize_trace(T, lnf, 0)
         for n, d in steps:
             T = hotrg.step3d(T, chi, d, pol(n))
             hotrg.normalize_trace(T, lnf, n)
+            if what:
+                torch.cuda.current_stream().synchronize()
+                log(f"{what}: step {n} {d} done")
         return T, lnf
 
     autotune.enable(not args.no_tune)
     t0 = time.perf_counter()
-    run(sched[:max(args.fp64_steps, 3) + min(3, args.warmup + 1)], policy, T0)
+    # the merge GEMMs run in sub-batches of at most 16 instances (hotrg.MERGE_BUDGET), so a
+    # 16-instance warm-up meets (and tunes) the same chi^11 problems as the timed solve
+    run(sched[:max(args.fp64_steps, 3) + min(3, args.warmup + 1)], policy, T0[:min(nb, 16)], "warm-up")
     torch.cuda.synchronize()
     autotune.enable(False)
     log(f"warm-up in {time.perf_counter() - t0:.1f} s")
@@ -215,7 +220,7 @@ def bench_hotrg3d(args):
     extra = {"lnZ_per_site": {"betas": betas[:3], "values": zt[:3].tolist()}}
     if not args.no_validate:
         sel = [0, 1, nb - 1] if nb > 2 else list(range(nb))
-        Td, ld = run(sched, hotrg.ALL_FP64, ops.contiguous(T0[sel]))
+        Td, ld = run(sched, hotrg.ALL_FP64, ops.contiguous(T0[sel]), "float64 validation")
         zd = hotrg.lnz3d(Td, ld, K).cpu().numpy()
         errs = [abs(float(zt[i]) - float(zd[j])) for j, i in enumerate(sel)]
         extra["lnZ_err_vs_fp64"] = max(errs)

@@ -210,7 +210,7 @@ int trg_scale_rows(int dtype, int batch, long long n, long long period, long lon
 
 /*
  * Batched one-sided Jacobi SVD of m x n float64 matrices (n <= m <= 256,
- * m*n*8 + 16 KiB within the LDS of one workgroup).  A (stride sA,
+ * m*(n|1)*8 + 16 KiB within the LDS of one workgroup).  A (stride sA,
  * row-major) is read only; s: singular values descending (stride n), U:
  * m x n (stride m*n), V: n x n (stride n*n, right singular vectors as
  * columns).  Replaces cupy.linalg.svd(full_matrices=False) of the small

@@ -188,36 +188,86 @@ def squeezer_environments(A, UM, sM, VMH, D):
 
 
 EIG_Q = 12   # power iterations of the top-k SVD of RX Rx when it is too large for the dense path
+# Run the squeezer's independent decompositions (four Cholesky roots, two SVDs)
+# as batched launches.  Every kernel is batch-invariant, so each result is the
+# same bits either way; False reproduces the reference's one-at-a-time order.
+BATCH_DECOMPOSITIONS = True
+
+
+def _by_shape(fn, mats):
+    """fn applied to each matrix of `mats`; with BATCH_DECOMPOSITIONS the
+    matrices of equal shape are stacked into one batched call.  fn maps a
+    (B, m, n) stack to a tuple of (B, ...) stacks (or one stack)."""
+    if not BATCH_DECOMPOSITIONS:
+        res = [fn(m[None]) for m in mats]
+        return [tuple(x[0] for x in r) if isinstance(r, tuple) else r[0] for r in res]
+    groups = {}
+    for i, m in enumerate(mats):
+        groups.setdefault(tuple(m.shape), []).append(i)
+    out = [None] * len(mats)
+    for idx in groups.values():
+        if len(idx) == 1:
+            stack = mats[idx[0]][None]
+        else:
+            stack = torch.empty((len(idx),) + tuple(mats[idx[0]].shape), dtype=mats[idx[0]].dtype,
+                                device=mats[idx[0]].device)
+            for j, i in enumerate(idx):
+                ops.copy_into(stack[j], mats[i])
+        r = fn(stack)
+        for j, i in enumerate(idx):
+            out[i] = tuple(x[j] for x in r) if isinstance(r, tuple) else r[j]
+    return out
+
+
+def _oblique_pairs(items, Dcut, dt, deg_eps=None, trunc_eps=None):
+    """:278-292 for each (Env_big, Env_small, shape_big, shape_small) of `items`, with
+    R^T R = Env_big (RX) and Rx Rx^T = Env_small; returns [(Px, PX), ...].  The
+    Cholesky roots, the products RX Rx and their SVDs of all items run as
+    batched launches (BATCH_DECOMPOSITIONS).  The rank rule uses this module's
+    DGENERACY_EPS / truncate_eps unless deg_eps / trunc_eps are given."""
+    deg_eps = degeneracy_eps if deg_eps is None else deg_eps
+    trunc_eps = truncate_eps if trunc_eps is None else trunc_eps
+    roots = _by_shape(linalg.chol_psd_upper, [e for it in items for e in (it[0], it[1])])
+    RXs = roots[0::2]                                                  # R^T R = XdagX
+    Rxs = [ops.contiguous(r.transpose(0, 1)) for r in roots[1::2]]    # Rx Rx^T = xxdag
+    Ps = [ops.gemm(RX, Rx) for RX, Rx in zip(RXs, Rxs)]
+    k1s = [min(RX.shape[0], Rx.shape[1], Dcut) for RX, Rx in zip(RXs, Rxs)]
+    small = [linalg._fits_small(P.shape[0], P.shape[0]) for P in Ps]
+    svds = [None] * len(Ps)
+    dense = [i for i in range(len(Ps)) if small[i]]
+    for i, r in zip(dense, _by_shape(linalg.svd_small, [Ps[i] for i in dense])):
+        svds[i] = r
+    topk = {}
+    for i in range(len(Ps)):
+        if not small[i]:
+            n = Ps[i].shape[0]
+            topk.setdefault(min(n, max(2 * k1s[i], k1s[i] + 8), linalg.SMALL_MAX), []).append(i)
+    for l, idx in topk.items():
+        for i, r in zip(idx, _by_shape(lambda A: linalg.svd_topk(A, l, EIG_Q), [Ps[i] for i in idx])):
+            svds[i] = r
+    out = []
+    for i, (Env_big, Env_small, shape_big, shape_small) in enumerate(items):
+        RX, Rx, P, k1 = RXs[i], Rxs[i], Ps[i], k1s[i]
+        u, s, vh = svds[i]
+        n = P.shape[0]
+        if small[i]:
+            chi, _ = linalg.truncate(s, k1, deg_eps, trunc_eps)
+        else:
+            chi, _ = linalg.truncate_with_tail(s, P, k1, deg_eps, trunc_eps)
+        chi = min(chi, Dcut)
+        sis = ops.contiguous(s[:chi])
+        v = ops.contiguous(vh[:chi].transpose(0, 1))                  # (n, chi)
+        PX = ops.gemm(Rx, v)
+        ops.scale_rows_(PX, sis, chi, 1, 2)                            # * s^-1/2 (columns)
+        Px = ops.gemm(ops.contiguous(u[:, :chi].transpose(0, 1)), RX)
+        ops.scale_rows_(Px, sis, chi, n, 2)                            # s^-1/2 * (rows)
+        out.append((ops.cast(Px, dt).view((chi,) + shape_small), ops.cast(PX, dt).view(shape_big + (chi,))))
+    return out
 
 
 def _oblique_pair(Env_big, Env_small, Dcut, shape_big, shape_small, dt, deg_eps=None, trunc_eps=None):
-    """:278-292 with R^T R = Env_big (RX) and Rx Rx^T = Env_small; returns Px, PX.
-    The rank rule uses this module's DGENERACY_EPS / truncate_eps unless
-    deg_eps / trunc_eps are given."""
-    deg_eps = degeneracy_eps if deg_eps is None else deg_eps
-    trunc_eps = truncate_eps if trunc_eps is None else trunc_eps
-    RX = linalg.chol_psd_upper(Env_big[None])[0]                     # R^T R = XdagX
-    Rx = ops.contiguous(linalg.chol_psd_upper(Env_small[None])[0].transpose(0, 1))   # Rx Rx^T = xxdag
-    P = ops.gemm(RX, Rx)
-    n = P.shape[0]
-    k1 = min(RX.shape[0], Rx.shape[1], Dcut)
-    if linalg._fits_small(n, n):
-        u, s, vh = linalg.svd_small(P[None])
-        u, s, vh = u[0], s[0], vh[0]
-        chi, _ = linalg.truncate(s, k1, deg_eps, trunc_eps)
-    else:
-        l = min(n, max(2 * k1, k1 + 8), linalg.SMALL_MAX)
-        u, s, vh = linalg.svd_topk(P[None], l, EIG_Q)
-        u, s, vh = u[0], s[0], vh[0]
-        chi, _ = linalg.truncate_with_tail(s, P, k1, deg_eps, trunc_eps)
-    chi = min(chi, Dcut)
-    sis = ops.contiguous(s[:chi])
-    v = ops.contiguous(vh[:chi].transpose(0, 1))                      # (n, chi)
-    PX = ops.gemm(Rx, v)
-    ops.scale_rows_(PX, sis, chi, 1, 2)                                # * s^-1/2 (columns)
-    Px = ops.gemm(ops.contiguous(u[:, :chi].transpose(0, 1)), RX)
-    ops.scale_rows_(Px, sis, chi, n, 2)                                # s^-1/2 * (rows)
-    return (ops.cast(Px, dt).view((chi,) + shape_small), ops.cast(PX, dt).view(shape_big + (chi,)))
+    """One oblique projector pair (:278-292); see _oblique_pairs."""
+    return _oblique_pairs([(Env_big, Env_small, shape_big, shape_small)], Dcut, dt, deg_eps, trunc_eps)[0]
 
 
 def squeezer(A, UM, sM, VMH, D, Dcut):
@@ -231,8 +281,9 @@ def squeezer(A, UM, sM, VMH, D, Dcut):
     eigenvalues, amplified by s^-1/2, make a float32 run diverge."""
     dt = A.dtype
     XdX, YdY, xxd, yyd = squeezer_environments(_f64(A), _f64(UM), sM, _f64(VMH), _f64(D))
-    Px, PX = _oblique_pair(XdX, xxd, Dcut, (A.shape[1], UM.shape[1]), (VMH.shape[2], D.shape[2]), dt)
-    Py, PY = _oblique_pair(YdY, yyd, Dcut, (A.shape[2], UM.shape[2]), (VMH.shape[3], D.shape[3]), dt)
+    (Px, PX), (Py, PY) = _oblique_pairs([(XdX, xxd, (A.shape[1], UM.shape[1]), (VMH.shape[2], D.shape[2])),
+                                         (YdY, yyd, (A.shape[2], UM.shape[2]), (VMH.shape[3], D.shape[3]))],
+                                        Dcut, dt)
     return Px, PX, Py, PY
 
 
@@ -241,23 +292,26 @@ def squeezer(A, UM, sM, VMH, D, Dcut):
 # ---------------------------------------------------------------------------
 def contract_G(A, PX, UM, PY, compute=None):
     """G_{T,X,Y,k} = A_{T,a,b,i} PX_{a,c,X} UM_{i,c,d,k} PY_{b,d,Y} (:379-384)."""
-    W = contract("Tabi,acX->TbicX", A, PX, compute=compute)
-    W = contract("TbicX,icdk->TbXdk", W, UM, compute=compute)
-    return contract("TbXdk,bdY->TXYk", W, PY, compute=compute)
+    W = contract("Tabi,acX->TbicX", A, PX, compute=compute, tag="atrg_cg")
+    W = contract("TbicX,icdk->TbXdk", W, UM, compute=compute, tag="atrg_cg")
+    return contract("TbXdk,bdY->TXYk", W, PY, compute=compute, tag="atrg_cg")
 
 
 def contract_H(VMH, Px, D, Py, compute=None):
     """H_{k,t,x,y} = VMH_{k,j,a,b} Px_{x,a,c} D_{j,t,c,d} Py_{y,b,d} (:400-405)."""
-    W = contract("kjab,xac->kjbxc", VMH, Px, compute=compute)
-    W = contract("kjbxc,jtcd->kbxtd", W, D, compute=compute)
-    return contract("kbxtd,ybd->ktxy", W, Py, compute=compute)
+    W = contract("kjab,xac->kjbxc", VMH, Px, compute=compute, tag="atrg_cg")
+    W = contract("kjbxc,jtcd->kbxtd", W, D, compute=compute, tag="atrg_cg")
+    return contract("kbxtd,ybd->ktxy", W, Py, compute=compute, tag="atrg_cg")
+
+
+def _trunc_triplet(u, s, vh, k):
+    chi, _ = linalg.truncate(s, k, degeneracy_eps, truncate_eps)
+    return ops.contiguous(u[:, :chi]), ops.contiguous(s[:chi]), ops.contiguous(vh[:chi])
 
 
 def _svd_trunc(M2, k):
     u, s, vh = linalg.svd_thin(_f64(M2)[None])
-    u, s, vh = u[0], s[0], vh[0]
-    chi, _ = linalg.truncate(s, k, degeneracy_eps, truncate_eps)
-    return ops.contiguous(u[:, :chi]), ops.contiguous(s[:chi]), ops.contiguous(vh[:chi])
+    return _trunc_triplet(u[0], s[0], vh[0], k)
 
 
 def coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY, compute=None):
@@ -267,8 +321,16 @@ def coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY, compute=None):
     nk = UM.shape[3]
     G = contract_G(A, PX, UM, PY, compute)
     H = contract_H(VMH, Px, D, Py, compute)
-    UG, sG, VGH = _svd_trunc(G.reshape(nT * nX * nY, nk), min(nT * nX * nY, nk))
-    UH, sH, VHH = _svd_trunc(H.reshape(nk, nt * nx * ny), min(nk, nt * nx * ny))
+    Gm, Hm = G.reshape(nT * nX * nY, nk), H.reshape(nk, nt * nx * ny)
+    if BATCH_DECOMPOSITIONS and Gm.shape == Hm.shape[::-1] and Gm.shape[0] >= Gm.shape[1]:
+        # the thin SVDs of G and of H^T share one batched launch sequence
+        # (svd(H) is the transpose of svd(H^T))
+        (uG, sGa, vhG), (uHt, sHa, vhHt) = _by_shape(linalg.svd_thin, [_f64(Gm), _f64(Hm).transpose(0, 1)])
+        UG, sG, VGH = _trunc_triplet(uG, sGa, vhG, min(nT * nX * nY, nk))
+        UH, sH, VHH = _trunc_triplet(vhHt.transpose(0, 1), sHa, uHt.transpose(0, 1), min(nk, nt * nx * ny))
+    else:
+        UG, sG, VGH = _svd_trunc(Gm, min(nT * nX * nY, nk))
+        UH, sH, VHH = _svd_trunc(Hm, min(nk, nt * nx * ny))
     # K = sG VGH sM UH sH
     Kl = ops.contiguous(VGH)
     ops.scale_rows_(Kl, sG, sG.numel(), Kl.shape[1], 0)

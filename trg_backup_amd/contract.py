@@ -50,7 +50,7 @@ def _as_matrix(t, labels, batch, rows, cols):
     return torch.as_strided(t, (nb, nr, nc), (sb, sr, sc), t.storage_offset())
 
 
-def contract(spec, A, B, compute=None):
+def contract(spec, A, B, compute=None, tag=None):
     ins, out = spec.replace(" ", "").split("->")
     la, lb = ins.split(",")
     if len(set(la)) != len(la) or len(set(lb)) != len(lb):
@@ -85,7 +85,7 @@ def contract(spec, A, B, compute=None):
     Cm = C.view(nb, Am.shape[1], Bm.shape[2])
     if Am.shape[0] == 1 and nb > 1:
         Am = Am.expand(nb, -1, -1)
-    ops.gemm(Am, Bm, Cm, compute=compute)
+    ops.gemm(Am, Bm, Cm, compute=compute, tag=tag)
     produced = batch + fa + fb
     if produced == list(out):
         return C

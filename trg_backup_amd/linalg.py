@@ -44,7 +44,7 @@ def svd_small(A, batch=None):
 
 
 def _fits_small(m, n):
-    return n <= m <= 256 and m * n * 8 + 16384 <= 163840
+    return n <= m <= 256 and m * (n | 1) * 8 + 16384 <= 163840
 
 
 def orthonormalize(X, perturb=PERTURB, stream_id=7, passes=3, return_info=False):
