@@ -510,3 +510,74 @@ def two_point_function(T, T0, Tn, Dcut, TOT_RGSTEPS):
         T0.loc[direction] //= 2
     lnz, (G,) = field_expected_value(T, [T0], total)
     return T, T0, lnz, G
+
+
+# ---------------------------------------------------------------------------
+# diagnostics (trg/ATRG_3d_new.py:768-951); the tensor is T_{TXYtxy} = U s VH
+# ---------------------------------------------------------------------------
+def find_max(T, max_nth):
+    """:768-816: the max_nth entries of the full tensor with the largest |value|
+    and their 6-leg indices (T, X, Y, t, x, y).  The reference prints them and
+    calls sys.exit(0) before returning; this returns them.  Evaluated by the
+    same (x, y) slices, each Tsub[T,X,Y,t] = sum_i U s VH[:, :, x, y]."""
+    chiT, chiX, chiY, _ = T.U.shape
+    vals, idx = [], []
+    for ix in range(chiX):
+        for iy in range(chiY):
+            sub = np.einsum("TXYi,i,it->TXYt", T.U, T.s, T.VH[:, :, ix, iy]).ravel()
+            vals.append(sub)
+            k = np.arange(sub.size)
+            TT, XX, YY, tt = np.unravel_index(k, (chiT, chiX, chiY, chiT))
+            idx.append(np.ravel_multi_index((TT, XX, YY, tt, np.full_like(k, ix), np.full_like(k, iy)),
+                                            (chiT, chiX, chiY, chiT, chiX, chiY)))
+    vals = np.concatenate(vals)
+    idx = np.concatenate(idx)
+    order = np.argsort(-np.abs(vals), kind="stable")[:max_nth]
+    ind = np.array(np.unravel_index(idx[order], (chiT, chiX, chiY, chiT, chiX, chiY))).T
+    return vals[order], ind
+
+
+def tensor_diagonal(T):
+    """T[i,j,k,i,j,k] = sum_a U[i,j,k,a] s_a VH[a,i,j,k] (:836-838)."""
+    return np.einsum("ijka,a,aijk->ijk", T.U, T.s, T.VH).ravel()
+
+
+def find_1stto3rdmax(T):
+    """:818-852: the three diagonal entries of largest |value|."""
+    d = tensor_diagonal(T)
+    return d[np.argsort(-np.abs(d), kind="stable")[:3]]
+
+
+def density_matrix(T, leg):
+    """The partial traces of :856 and :884-923: leg "t" -> Tt (trace over x, y),
+    "x" -> Xx (over t, y), "y" -> Yy (over t, x)."""
+    spec = {"t": "TXYi,i,itXY->Tt", "x": "TXYi,i,iTxY->Xx", "y": "TXYi,i,iTXy->Yy"}[leg]
+    return np.einsum(spec, T.U, T.s, T.VH)
+
+
+def cal_scaling_dimensions(T):
+    """:854-877: eigh of the T-density matrix (the lower triangle, like
+    cupy.linalg.eigh), descending, floored at 1e-16, padded to Dcut,
+    log(e0 / e) / (2 pi)."""
+    e = np.linalg.eigh(density_matrix(T, "t"), UPLO="L")[0][::-1].copy()
+    e[e < 1e-16] = 1e-16
+    e = np.pad(e, (0, T.Dcut - len(e)), constant_values=1e-16)
+    return np.log(np.max(e) / e) / (2 * np.pi)
+
+
+def cal_coorelation(T, rgstep):
+    """:879-951: singular values of the three density matrices normalised by
+    their largest, and ln xi = n_leg ln 2 - ln(-1 / ln e_1) per leg.  As in the
+    reference, the y branch tests the x leg's e_1 against 1e-200 (:944)."""
+    out = {}
+    for leg, key in (("t", "T"), ("x", "X"), ("y", "Y")):
+        e = np.linalg.svd(density_matrix(T, leg), compute_uv=False)
+        if len(e) < T.Dcut:
+            e = np.pad(e, (0, T.Dcut - len(e)))
+        out[leg] = e / np.max(e)
+    lnxi = {}
+    for leg, key in (("t", "T"), ("x", "X"), ("y", "Y")):
+        cond = out["x"][1] if leg == "y" else out[leg][1]
+        e1 = out[leg][1] if cond > 1e-200 else 1e-200
+        lnxi[leg] = rgstep[key] * np.log(2) - np.log(-1 / np.log(e1))
+    return out, lnxi

@@ -25,8 +25,10 @@ def _steady(B, chi, gpu, prec):
     return ops.cast(T, prec.store), lnf, n, d
 
 
-@pytest.mark.parametrize("precname", ["FP64", "FP32"])
-def test_graph_replays_equal_eager_steps(gpu, precname):
+@pytest.mark.parametrize("precname,shared", [("FP64", False), ("FP32", False), ("FP32", True)])
+def test_graph_replays_equal_eager_steps(gpu, precname, shared):
+    """Replays equal eager steps, also when the Y and X graphs share one
+    workspace pool."""
     from trg_backup_amd import graphs, hotrg, ops
     prec = getattr(hotrg, precname)
     B, chi = 3, 16
@@ -34,8 +36,9 @@ def test_graph_replays_equal_eager_steps(gpu, precname):
     lnf_e = ops.contiguous(lnf0)
     lnf_g = ops.contiguous(lnf0)
     counter = graphs.step_counter(gpu)
-    gY = graphs.GraphedStep2D(B, chi, "Y", prec, lnf_g, counter)
-    gX = graphs.GraphedStep2D(B, chi, "X", prec, lnf_g, counter)
+    pool = torch.cuda.graph_pool_handle() if shared else None
+    gY = graphs.GraphedStep2D(B, chi, "Y", prec, lnf_g, counter, pool=pool)
+    gX = graphs.GraphedStep2D(B, chi, "X", prec, lnf_g, counter, pool=pool)
     graphs.set_step(counter, n0 + 1)
     Te, Tg, d = ops.contiguous(T0), ops.contiguous(T0), d0
     for n in range(n0 + 1, n0 + 7):

@@ -30,6 +30,10 @@ degeneracy_eps = 0.0
 truncate_eps = 1e-10
 # leg-swap rSVD parameters (reference: k_factor 3, p 0, q = Dcut, :193)
 RSVD = {"k_factor": 3, "p": 0, "q": None, "seed": 12345}
+# called as hook(sM, k, ispure) with the leg-swap singular values of every
+# step (k = max(Dcut, chi_k)); the drop-in writes the reference's
+# leg_exchange_*/ files through it (trg/ATRG_3d_new.py:197-212)
+LEG_SWAP_HOOK = None
 
 
 def set_rsvd_params(k_factor=3, p=0, q=None, seed=12345):
@@ -151,6 +155,8 @@ def leg_swapping(B, C, Dcut, ispure=True, prec="f64"):
     q = Dcut if RSVD["q"] is None else RSVD["q"]
     UM, sM, VMH = rsvd_for_ATRG_leg_swapping(Bt, Ct, RSVD["k_factor"] * chi_k, RSVD["p"], q, truncate_eps,
                                              RSVD["seed"], keep=chi_k)
+    if LEG_SWAP_HOOK is not None:
+        LEG_SWAP_HOOK(sM, max(Dcut, chi_k), ispure)
     chi, _ = linalg.truncate(sM, chi_k, degeneracy_eps, truncate_eps)
     UM = ops.contiguous(UM[:, :, :chi]).view(ni, nX0, nY0, chi)
     VMH = ops.contiguous(VMH[:chi]).view(chi, nj, nx1, ny1)

@@ -33,9 +33,13 @@ def set_step(counter, n):
 
 class GraphedStep2D:
     """`health` (an ops.Health, optional) receives the breakdown flags and bad
-    norms of every replay; check() reads it."""
+    norms of every replay; check() reads it.  Graphs that are replayed one
+    after another on one stream may share a memory `pool`
+    (torch.cuda.graph_pool_handle()): each step's intermediates are dead once
+    it ends, so the steps of a cycle reuse one another's workspace and only
+    their outputs stay resident."""
 
-    def __init__(self, batch, chi, direction, prec, lnf, step, warmup=1, health=None):
+    def __init__(self, batch, chi, direction, prec, lnf, step, warmup=1, health=None, pool=None):
         dev = lnf.device
         self.batch, self.chi, self.direction, self.prec = batch, chi, direction, prec
         self.inp = torch.empty((batch,) + (chi,) * getattr(self, "ndim", 4), dtype=prec.store, device=dev)
@@ -53,7 +57,7 @@ class GraphedStep2D:
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph), ops.isolated_health(self.health):
+        with torch.cuda.graph(self.graph, pool=pool), ops.isolated_health(self.health):
             self.out = self._body(lnf, step)
         torch.cuda.synchronize()
 
@@ -76,9 +80,9 @@ class GraphedStep2D:
 class GraphedStep3D(GraphedStep2D):
     """A captured steady-state batched 3D HOTRG step (legs chi^6)."""
 
-    def __init__(self, batch, chi, direction, prec, lnf, step, warmup=1, health=None):
+    def __init__(self, batch, chi, direction, prec, lnf, step, warmup=1, health=None, pool=None):
         self.ndim = 6
-        super().__init__(batch, chi, direction, prec, lnf, step, warmup, health)
+        super().__init__(batch, chi, direction, prec, lnf, step, warmup, health, pool)
 
     def _body(self, lnf, step):
         T = hotrg.step3d(self.inp, self.chi, self.direction, self.prec)

@@ -135,3 +135,16 @@ def internal_energy(beta, h, Dcut, XLOOPS, YLOOPS, TLOOPS, gilt_eps=0):
         lnz, G = a.cal_field_expected_value(T, [T0], steps)
         return lnz, G[0]
     raise ValueError("Only support hotrg")
+
+
+def write_result(outputname, beta, h, lnZ_OVER_V, observable=None):
+    """One line of the 3D Ising driver's result file (Ising_model_3d.py:40-42,
+    :51-55): "beta h lnZ/V" or "beta h lnZ/V m", each value of lnZ/V and m in
+    the driver's {:19.12e} form (real parts), written fresh like the driver's
+    open(outputname, "w")."""
+    line = "{:} {:} {:19.12e}".format(beta, h, float(np.real(lnZ_OVER_V)))
+    if observable is not None:
+        line += " {:19.12e}".format(float(np.real(observable)))
+    with open(outputname, "w") as out:
+        out.write(line + "\n")
+    return line

@@ -3,13 +3,14 @@
 Same names as trg/ATRG_3d_new.py:18-482 and 954-1039.  ``pure_tensor``
 takes an ATRG_Tensor (U, s, VH as NumPy arrays or GPU tensors) and returns it
 coarse grained, with normal_const filled as in :967-972.  Environment:
-DGENERACY_EPS (default 0.0, the reference requires it, :15); the flows do
-not write the per-step files under OUTPUT_DIR (:197-212, :736-766);
-save_singularvalues and cal_X write theirs when called.  The
-one-point (:614-734, :1043-1189) and two-point (:485-612, :1333-1489)
-impurity flows run on the device with the pure tensor; the scaling-dimension
-diagnostics and the one_point_function1 variant (:768-951, :1191-1331) are
-not on the device and raise NotImplementedError.
+DGENERACY_EPS (default 0.0, the reference requires it, :15) and OUTPUT_DIR:
+when it is set, pure_tensor writes the reference's per-step files
+(tensor_sv/*.dat, X.dat, leg_exchange_pure/*.dat; :197-212, :736-766,
+:991-993, :1023-1025); without it nothing is written.  The one-point
+(:614-734, :1043-1189, and the single-impurity form :1191-1331) and two-point
+(:485-612, :1333-1489) impurity flows run on the device with the pure tensor,
+as do the diagnostics find_max, find_1stto3rdmax, cal_scaling_dimensions and
+cal_coorelation (:768-951).
 ``set_precision("f32")`` runs the contractions in float32 on MFMA;
 decompositions stay float64.
 """
@@ -20,6 +21,7 @@ import os
 import torch
 
 from .. import atrg, linalg, ops
+from ..contract import contract
 from .._io import from_device, to_device
 from .._lib import TrgError
 
@@ -152,25 +154,60 @@ def _log_norm(c, nsteps):
     return math.log(v) / 2 ** nsteps
 
 
+def _leg_exchange_writer(out_dir):
+    """The leg-swap spectrum file of trg/ATRG_3d_new.py:197-212: sM (the
+    reference divides by 1), zero-padded to 3 max(Dcut, chi_k), to
+    leg_exchange_{pure,impure}/leg_exchange_sv_lx{X}_ly{Y}_lt{T}.dat."""
+    def hook(sM, k, ispure):
+        d = os.path.join(out_dir, "leg_exchange_pure" if ispure else "leg_exchange_impure")
+        os.makedirs(d, exist_ok=True)
+        vals = [float(v) for v in sM.double().cpu().tolist()]
+        vals += [0.0] * max(0, 3 * k - len(vals))
+        with open(os.path.join(d, f"leg_exchange_sv_lx{rgstep['X']}_ly{rgstep['Y']}_lt{rgstep['T']}.dat"), "w") as f:
+            f.write(f"#s1={1.0:.12e}\n")
+            f.writelines(f"{v:.12e}\n" for v in vals)
+    return hook
+
+
+def _step_outputs(U, s, VH, Dcut):
+    """The per-step files of :991-993 and :1023-1025 (tensor_sv/*.dat and
+    X.dat) for the current device tensors; only with OUTPUT_DIR set."""
+    from ..tensor_class.tensor_class import ATRG_Tensor
+    t = ATRG_Tensor(U, s, VH, Dcut, 3, False, {})
+    save_singularvalues(t, Dcut)
+    cal_X(t, save=True)
+
+
 def pure_tensor(T, Dcut, TOT_RGSTEPS):
     """:974-1039: cycle T, X, Y until the totals are reached.  The tensor stays
     on the GPU between steps and is returned in the caller's array kind."""
     U, s, VH, kind = _dev3(T)
     rgstep = _reset_steps()
+    out_dir = os.environ.get("OUTPUT_DIR")
+    old_hook = atrg.LEG_SWAP_HOOK
+    if out_dir:
+        atrg.LEG_SWAP_HOOK = _leg_exchange_writer(out_dir)
     health = ops.Health(U.device)
-    with health:
-        s, c = atrg.normalize(U, s, VH)
-        T.normal_const[f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"] = _log_norm(c, 0)
-        total = sum(TOT_RGSTEPS.values())
-        for direction in itertools.cycle("TXY"):
-            if sum(rgstep.values()) >= total:
-                break
-            if rgstep[direction] < TOT_RGSTEPS[direction]:
-                rgstep[direction] += 1
-                U, s, VH = atrg.atrg_pure_tensor(U, s, VH, Dcut, direction, _step_precision(sum(rgstep.values())))
-                s, c = atrg.normalize(U, s, VH)
-                key = f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
-                T.normal_const[key] = _log_norm(c, sum(rgstep.values()))
+    try:
+        with health:
+            s, c = atrg.normalize(U, s, VH)
+            T.normal_const[f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"] = _log_norm(c, 0)
+            if out_dir:
+                _step_outputs(U, s, VH, Dcut)
+            total = sum(TOT_RGSTEPS.values())
+            for direction in itertools.cycle("TXY"):
+                if sum(rgstep.values()) >= total:
+                    break
+                if rgstep[direction] < TOT_RGSTEPS[direction]:
+                    rgstep[direction] += 1
+                    U, s, VH = atrg.atrg_pure_tensor(U, s, VH, Dcut, direction, _step_precision(sum(rgstep.values())))
+                    s, c = atrg.normalize(U, s, VH)
+                    key = f"({rgstep['X']},{rgstep['Y']},{rgstep['T']})"
+                    T.normal_const[key] = _log_norm(c, sum(rgstep.values()))
+                    if out_dir:
+                        _step_outputs(U, s, VH, Dcut)
+    finally:
+        atrg.LEG_SWAP_HOOK = old_hook
     health.check("ATRG solve")
     T.update(from_device(U, kind), from_device(s, kind), from_device(VH, kind))
     return T
@@ -335,20 +372,185 @@ def cal_X(T, save=False):
     return Xt, Xx, Xy
 
 
-def _not_on_device(name, lines):
-    def f(*args, **kwargs):
-        raise NotImplementedError(f"ATRG_3d_new.{name} (trg/ATRG_3d_new.py:{lines}) is not on the device")
-    f.__name__ = name
-    return f
+def _scaled_u(T):
+    """(U s (T,X,Y,i) float64, VH float64) of an ATRG tensor on the device."""
+    U, s, VH, _ = _dev3(T)
+    Us = ops.cast(U, torch.float64)
+    ops.scale_rows_(Us, ops.cast(s, torch.float64).reshape(-1), Us.shape[3], 1, 0)
+    return Us, ops.cast(VH, torch.float64)
 
 
-find_max = _not_on_device("find_max", "768-816")
-find_1stto3rdmax = _not_on_device("find_1stto3rdmax", "818-852")
-cal_scaling_dimensions = _not_on_device("cal_scaling_dimensions", "854-877")
-cal_coorelation = _not_on_device("cal_coorelation", "879-951")
-cal_field_expected_value1 = _not_on_device("cal_field_expected_value1", "1191-1222")
-exec_rgstep_1point_func1 = _not_on_device("exec_rgstep_1point_func1", "1224-1244")
-one_point_function1 = _not_on_device("one_point_function1", "1246-1331")
+def _density_matrix(T, leg):
+    """Partial traces of the tensor T_{TXYtxy} = U s VH (:856, :884-923):
+    leg "t" -> [T,t] (x, y traced), "x" -> [X,x], "y" -> [Y,y], on the device."""
+    Us, VH = _scaled_u(T)
+    spec = {"t": "TXYi,itXY->Tt", "x": "TXYi,iTxY->Xx", "y": "TXYi,iTXy->Yy"}[leg]
+    return contract(spec, Us, VH)
+
+
+def _append_mode():
+    return "w" if sum(rgstep.values()) == 0 else "a"
+
+
+def find_max(T, max_nth):
+    """:768-816: the max_nth entries of the full tensor T_{TXYtxy} = U s VH with the
+    largest |value| and their 6-leg indices.  The slices Tsub[T,X,Y,t] of each
+    (x, y) are contracted on the device and the running top entries are kept
+    on the host (the reference sorts each merged slice the same way).  The
+    reference prints and calls sys.exit(0) at :813-814; this returns
+    (values, indices) instead."""
+    import numpy as np
+    Us, VH = _scaled_u(T)
+    chiT, chiX, chiY, _ = Us.shape
+    shape6 = (chiT, chiX, chiY, VH.shape[1], chiX, chiY)
+    best_v = np.empty(0)
+    best_i = np.empty(0, dtype=np.int64)
+    for ix in range(chiX):
+        for iy in range(chiY):
+            sub = contract("TXYi,it->TXYt", Us, VH[:, :, ix, iy]).cpu().numpy().ravel()
+            k = np.arange(sub.size)
+            TT, XX, YY, tt = np.unravel_index(k, (chiT, chiX, chiY, VH.shape[1]))
+            flat = np.ravel_multi_index((TT, XX, YY, tt, np.full_like(k, ix), np.full_like(k, iy)), shape6)
+            v = np.concatenate((best_v, sub))
+            i = np.concatenate((best_i, flat))
+            order = np.argsort(-np.abs(v), kind="stable")[:max_nth]
+            best_v, best_i = v[order], i[order]
+    return best_v, np.array(np.unravel_index(best_i, shape6)).T
+
+
+def find_1stto3rdmax(T, save=False):
+    """:818-852: the three diagonal entries T[i,j,k,i,j,k] of largest |value|
+    (diagonal = sum_a U[i,j,k,a] s_a VH[a,i,j,k], contracted on the device);
+    with ``save`` one line goes to OUTPUT_DIR/max_1to3-th_values.dat (new file
+    at step 0).  Returns the three values."""
+    import numpy as np
+    Us, VH = _scaled_u(T)
+    d = contract("ijka,aijk->ijk", Us, VH).cpu().numpy().ravel()
+    top = d[np.argsort(-np.abs(d), kind="stable")[:3]]
+    top = np.pad(top, (0, 3 - len(top)))
+    out_dir = os.environ.get("OUTPUT_DIR")
+    if save and out_dir:
+        with open(os.path.join(out_dir, "max_1to3-th_values.dat"), _append_mode()) as f:
+            f.write(f"{top[0]:40.12e} {top[1]:40.12e} {top[2]:40.12e}\n")
+    return top
+
+
+def _lower_symmetric(A):
+    """The symmetric matrix of A's lower triangle (what cupy.linalg.eigh reads,
+    UPLO="L"), built with device copies."""
+    n = A.shape[0]
+    S = ops.contiguous(A)
+    for i in range(n - 1):
+        ops.copy_into(S[i, i + 1:], A[i + 1:, i])
+    return S
+
+
+def cal_scaling_dimensions(T, save=False):
+    """:854-877: eigenvalues of the t-density matrix (device Jacobi on its lower
+    triangle), descending, floored at 1e-16, padded to Dcut with 1e-16, and
+    log(e0 / e) / (2 pi); with ``save`` one line goes to
+    OUTPUT_DIR/scaling_dimensions.dat.  Returns the scaling dimensions."""
+    import numpy as np
+    M = _lower_symmetric(_density_matrix(T, "t"))
+    w, _ = linalg.eigh_small(M[None])
+    e = w[0].cpu().numpy()[::-1].copy()
+    e[e < 1e-16] = 1e-16
+    e = np.pad(e, (0, max(0, T.Dcut - len(e))), constant_values=1e-16)
+    sd = np.log(np.max(e) / e) / (2 * np.pi)
+    out_dir = os.environ.get("OUTPUT_DIR")
+    if save and out_dir:
+        with open(os.path.join(out_dir, "scaling_dimensions.dat"), _append_mode()) as f:
+            f.write(" ".join(f"{v:.12e}" for v in sd) + "\n")
+    return sd
+
+
+def cal_coorelation(T, save=False):
+    """:879-951: singular values of the t, x and y density matrices (device
+    Jacobi SVD) normalised by their largest, and ln xi = n ln 2 - ln(-1 / ln e_1)
+    per leg (n = this leg's RG steps); the y branch tests the x leg's e_1
+    against 1e-200, as the reference does (:944).  With ``save`` the files
+    lnξt.dat, lnξx.dat, lnξy.dat and density_matrix_t/DentMatT_*.dat are written
+    under OUTPUT_DIR.  Returns ({leg: e}, {leg: ln xi})."""
+    import numpy as np
+    ev = {}
+    for leg in ("t", "x", "y"):
+        M = _density_matrix(T, leg)
+        _, sv, _ = linalg.svd_small(M[None])
+        e = sv[0].cpu().numpy()
+        if len(e) < T.Dcut:
+            e = np.pad(e, (0, T.Dcut - len(e)))
+        ev[leg] = e / np.max(e)
+        if leg == "t":
+            etmax = float(np.max(e))
+    lnxi = {}
+    for leg, key in (("t", "T"), ("x", "X"), ("y", "Y")):
+        cond = ev["x"][1] if leg == "y" else ev[leg][1]
+        e1 = ev[leg][1] if cond > 1e-200 else 1e-200
+        lnxi[leg] = rgstep[key] * math.log(2) - math.log(-1 / math.log(e1))
+    out_dir = os.environ.get("OUTPUT_DIR")
+    if save and out_dir:
+        os.makedirs(out_dir, exist_ok=True)
+        mode = _append_mode()
+        for leg in ("t", "x", "y"):
+            with open(os.path.join(out_dir, f"lnξ{leg}.dat"), mode) as f:
+                f.write(f"{lnxi[leg]:.12e}\n")
+        dm = os.path.join(out_dir, "density_matrix_t")
+        os.makedirs(dm, exist_ok=True)
+        with open(os.path.join(dm, f"DentMatT_lx{rgstep['X']}_ly{rgstep['Y']}_lt{rgstep['T']}.dat"), mode) as f:
+            f.write(f"λ1={etmax:.12e}\n")
+            f.writelines(f"{v:.12e}\n" for v in ev["t"])
+    return ev, lnxi
+
+
+# ---------------------------------------------------------------------------
+# one-point flow with a single impurity tensor (:1191-1331)
+# ---------------------------------------------------------------------------
+def cal_field_expected_value1(T, T0, rgstep, save=False):
+    """:1191-1222: ln Z/V and P = exp(sum of T0's factors) Tr T0 / Tr T for one
+    impurity tensor; with ``save`` the line "nsteps lnZ/V P" goes to
+    OUTPUT_DIR/field_expected_value_rgsteps_dependence.dat.  Returns (lnZ/V, P)."""
+    lnz, P = cal_field_expected_value(T, [T0], rgstep, False)
+    n = sum(rgstep.values())
+    out_dir = os.environ.get("OUTPUT_DIR")
+    if save and out_dir:
+        with open(os.path.join(out_dir, "field_expected_value_rgsteps_dependence.dat"),
+                  "w" if n == 0 else "a") as f:
+            f.write(f"{n} {lnz:.12e} {P[0]:.12e}\n")
+    return lnz, P[0]
+
+
+def exec_rgstep_1point_func1(T, T0, Dcut, direction, rgstep):
+    """:1224-1244: one step of T and the single impurity T0, their
+    normalisations (factors ln c / 2^n and ln c0 - ln c) and T0's location."""
+    T, (T0,) = exec_rgstep_1point_func(T, [T0], Dcut, direction, rgstep)
+    return T, T0
+
+
+def one_point_function1(T, T0, Dcut, TOT_RGSTEPS):
+    """:1246-1331: the single-impurity flow, cycling T, X, Y; after the initial
+    normalisation and after every step it writes the singular values
+    (save_singularvalues) and the field expectation line
+    (cal_field_expected_value1, save=True) when OUTPUT_DIR is set, as the
+    reference does.  Returns (T, T0)."""
+    assert hasattr(T0, "U") and hasattr(T0, "loc"), "T0 must be ATRG_Tensor type"
+    step = _reset_steps()
+    key = f"({step['X']},{step['Y']},{step['T']})"
+    T, c = normalization(T)
+    T.normal_const[key] = math.log(c)
+    T0, c0 = normalization(T0)
+    T0.normal_const[key] = math.log(c0) - math.log(c)
+    save_singularvalues(T, Dcut)
+    cal_field_expected_value1(T, T0, step, True)
+    total = sum(TOT_RGSTEPS.values())
+    for direction in itertools.cycle("TXY"):
+        if sum(step.values()) >= total:
+            break
+        if step[direction] < TOT_RGSTEPS[direction]:
+            step[direction] += 1
+            T, T0 = exec_rgstep_1point_func1(T, T0, Dcut, direction, step)
+            save_singularvalues(T, Dcut)
+            cal_field_expected_value1(T, T0, step, True)
+    return T, T0
 
 
 # ---------------------------------------------------------------------------
