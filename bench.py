@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--atrg-q", type=int, default=2)
     ap.add_argument("--squeezer", default="QR", choices=["original", "QR", "fix"], help="hotrg3d_qr squeezer")
     ap.add_argument("--atrg-prec", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--atrg-deg-eps", type=float, default=0.0,
+                    help="atrg3d: DGENERACY_EPS of the rank rule (trg/ATRG_3d_new.py:15; the reference reads it "
+                         "from the environment with no default)")
     ap.add_argument("--merge3d", default="bf16x6", choices=["f32", "bf16x6"],
                     help="hotrg3d: compute type of the steady steps' GEMMs (bf16x6 = three-piece split-bf16, "
                          "fp32 accuracy)")
@@ -77,6 +80,10 @@ def parse():
     ap.add_argument("--split-merge-from", type=int, default=10, help="bf16x3 merge after this step (-1: never)")
     ap.add_argument("--bf16-env-from", type=int, default=12, help="bf16 environments after this step (-1: never)")
     ap.add_argument("--bf16-merge-from", type=int, default=22, help="bf16 merge after this step (-1: never)")
+    ap.add_argument("--h3-split-from", type=int, default=-1, help="hotrg3d: bf16x3 merges after this step (-1: never)")
+    ap.add_argument("--h3-bf16-from", type=int, default=-1, help="hotrg3d: bf16 merges after this step (-1: never)")
+    ap.add_argument("--h3-bf16-env-from", type=int, default=-1,
+                    help="hotrg3d: bf16 environments after this step (-1: never)")
     ap.add_argument("--validate-betas", type=int, default=0,
                     help="hotrg2d: validate this many couplings against float64 (0 = all of them)")
     ap.add_argument("--validate-all", action="store_true", help="validate every coupling (the hotrg2d default)")
@@ -139,13 +146,16 @@ def bench_hotrg3d(args):
     dev = torch.device("cuda:0")
     chi, nb, K = args.chi, args.nbeta, args.steps
     betas = workloads.hotrg3d_betas(nb)
-    policy = workloads.hotrg3d_policy(args.fp64_steps, args.merge3d)
+    policy = workloads.hotrg3d_policy(args.fp64_steps, args.merge3d, args.h3_split_from, args.h3_bf16_from,
+                                      args.h3_bf16_env_from)
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     T0 = torch.stack([models.initial_tensor("ising3d", b, dev) for b in betas])
     sched = workloads.schedule3d(K)
+    steady = workloads.tiers(policy, K, args.fp64_steps + 1)
 
     def run(steps, pol, T0_, what=None):
-        lnf = ops.fill_(torch.empty((T0_.shape[0], K + 1), dtype=torch.float64, device=dev), 0.0)
+        lnf = ops.fill_(torch.empty((T0_.shape[0], max(K, steps[-1][0]) + 1), dtype=torch.float64, device=dev),
+                        0.0)
         T = ops.contiguous(T0_, pol(0).store)
         hotrg.normalize_trace(T, lnf, 0)
         for n, d in steps:
@@ -159,16 +169,23 @@ def bench_hotrg3d(args):
     autotune.enable(not args.no_tune)
     t0 = time.perf_counter()
     # the merge GEMMs run in sub-batches of at most 16 instances (hotrg.MERGE_BUDGET), so a
-    # 16-instance warm-up meets (and tunes) the same chi^11 problems as the timed solve
-    run(sched[:max(args.fp64_steps, 3) + min(3, args.warmup + 1)], policy, T0[:min(nb, 16)], "warm-up")
+    # 16-instance warm-up meets (and tunes) the same chi^11 problems as the timed solve: the
+    # float64 steps, then three steps (Z, Y, X) in each steady precision tier
+    nfirst = max(args.fp64_steps, 3)
+    rep = list(range(nfirst + 1))
+    for n_t, _ in steady:
+        rep += [max(n_t, nfirst + 1)] * min(3, args.warmup + 1)
+    wsched = workloads.schedule3d(len(rep) - 1)
+    run(wsched, lambda n: policy(rep[n]), T0[:min(nb, 16)], "warm-up")
     torch.cuda.synchronize()
     autotune.enable(False)
     log(f"warm-up in {time.perf_counter() - t0:.1f} s")
     torch.cuda.empty_cache()
     graphed = {}
     health = ops.Health(dev)
-    if not args.no_graphs and K > args.fp64_steps:
-        # one captured batched launch sequence per steady direction (float32 steps)
+    if not args.no_graphs and K > args.fp64_steps and len(steady) == 1:
+        # one captured batched launch sequence per steady direction (float32 steps; a ladder
+        # of several steady tiers runs eagerly: a batched chi = 16 step is seconds of GPU work)
         from trg_backup_amd import graphs
         counter = graphs.step_counter(dev)
         lnf_g = ops.fill_(torch.empty((nb, K + 1), dtype=torch.float64, device=dev), 0.0)
@@ -264,10 +281,12 @@ def bench_hotrg3d(args):
     cfg = {"workload": f"3D Ising HOTRG chi={chi}, {K} RG steps (Z,Y,X) from the initial tensor, {nb} beta "
                        "from default_rng(0).uniform(0.20,0.24) batched",
            "chi": chi, "nbeta": nb, "rg_steps": K, "policy": policy.describe(), "parallelism": "single GPU, batched"}
-    if args.merge3d == "bf16x6":
-        roof = _roof(rec, "merge chi^11 GEMM (T.U1.U2 x T, bf16x6: six bf16 MFMAs per product)", PEAK_TFLOPS["bf16x6"])
+    last = policy(K).merge
+    if last in ("bf16", "bf16x3", "bf16x6"):
+        roof = _roof(rec, f"merge chi^11 GEMM (T.U1.U2 x T, {last}: {MFMA_PER_PRODUCT[last]} bf16 MFMAs per product, "
+                          f"last tier of the solve)", PEAK_TFLOPS[last])
         if roof:
-            roof["mfma_issue_frac"] = roof["frac"] * MFMA_PER_PRODUCT["bf16x6"]
+            roof["mfma_issue_frac"] = roof["frac"] * MFMA_PER_PRODUCT[last]
     else:
         roof = _roof(rec, "merge chi^11 GEMM (T.U1.U2 x T, f32 MFMA)", PEAK_TFLOPS["f32"])
     print(json.dumps(_line(cfg, nb * K / dt, K, args.warmup, dt, dtype_label(policy, K), roof, extra)), flush=True)
@@ -354,7 +373,8 @@ def bench_atrg3d(args):
     _lib.load()
     chi, K = args.chi, args.steps
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
-    saved = (dict(atrg.RSVD), atrg.BATCH_DECOMPOSITIONS)
+    saved = (dict(atrg.RSVD), atrg.BATCH_DECOMPOSITIONS, atrg.degeneracy_eps)
+    atrg.degeneracy_eps = args.atrg_deg_eps
 
     def solve(nsteps, prec, q, fp64_steps, what):
         atrg.set_rsvd_params(k_factor=1, p=chi, q=q, seed=1)
@@ -402,6 +422,7 @@ def bench_atrg3d(args):
     finally:
         atrg.RSVD.update(saved[0])
         atrg.BATCH_DECOMPOSITIONS = saved[1]
+        atrg.degeneracy_eps = saved[2]
     # coarse-graining GEMMs (G and H, 4 chi^7 flop per step) of the steps after the float64 ones
     names = {_lib.F32: "f32", _lib.F64: "f64", _lib.BF16: "bf16", _lib.BF16X3: "bf16x3", _lib.BF16X6: "bf16x6"}
     evs = [(e0.elapsed_time(e1), fl, cfg) for (e0, e1, fl), cfg in zip(rec.events, rec.event_configs)]
@@ -426,6 +447,7 @@ def bench_atrg3d(args):
                         "SVDs, PSD Cholesky roots), latency-bound; see the rocprof summary in profiles/"}
     cfg = {"workload": f"3D q=3 Potts ATRG chi={chi}, beta=0.55, {K} RG steps (T,X,Y), rSVD k=chi p=chi "
                        f"q={args.atrg_q} seed 1", "chi": chi, "rg_steps": K,
+           "DGENERACY_EPS": args.atrg_deg_eps,
            "precision": f"f64 for steps <= {args.fp64_steps}, then {args.atrg_prec} contractions "
                         "(decompositions and squeezer environments f64)",
            "parallelism": "single GPU, one instance, independent decompositions batched"}

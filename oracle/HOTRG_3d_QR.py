@@ -27,8 +27,9 @@ Decided reference behaviour (documented):
   arguments (`degeneracy_eps`, `squeezer_type`) and `slicing` is a
   parameter of `coarse_graining` that does not change the result.
 * Real tensors only: the reference conjugates, which is the identity here.
-* The per-step diagnostics (cal_X, cal_coorelation, cal_scaling_dimensions,
-  prints, timing) are dropped.
+* The per-step diagnostics (cal_X, find_1stto3rdmax, cal_scaling_dimensions,
+  cal_coorelation, :559-713) are functions of the tensor below that return
+  their values; the prints, timings and files are dropped.
 """
 import math
 from itertools import cycle, product
@@ -399,3 +400,67 @@ def two_point_function(T, T0, Tn, Dcut, TOT_RGSTEPS, squeezer_type="QR", degener
     G = math.exp(sum(T0["factor"].values())) * float(np.einsum("xxyyzz", T0["Tensor"])) \
         / float(np.einsum("xxyyzz", T["Tensor"]))
     return T, T0, G
+
+
+# ---------------------------------------------------------------------------
+# diagnostics (trg/HOTRG_3d_QR.py:559-713); T has legs (x, X, y, Y, t, T)
+# ---------------------------------------------------------------------------
+def density_matrix(T, leg):
+    """The partial traces the diagnostics use, in the reference's output order:
+    "t" -> "xxyytT->Tt", "x" -> "xXyytt->Xx", "y" -> "xxyYtt->Yy" (:617, :645,
+    :673, :695)."""
+    spec = {"t": "xxyytT->Tt", "x": "xXyytt->Xx", "y": "xxyYtt->Yy"}[leg]
+    return np.einsum(spec, T)
+
+
+def cal_X(T):
+    """:559-583: X_t, X_x, X_y = (Tr T)^2 / TT, TT the trace of two tensors
+    stacked along the leg ("xxyytT,aabbTt", "xXyytt,Xxaabb", "xxyYtt,aaYybb")."""
+    tr = np.einsum("xxyytt", T)
+    TTt = np.einsum("xxyytT,aabbTt", T, T)
+    TTx = np.einsum("xXyytt,Xxaabb", T, T)
+    TTy = np.einsum("xxyYtt,aaYybb", T, T)
+    return tr ** 2 / TTt, tr ** 2 / TTx, tr ** 2 / TTy
+
+
+def find_1stto3rdmax(T):
+    """:585-611: the three diagonal entries T[x,x,y,y,t,t] of largest |value|
+    (the reference writes them and returns nothing)."""
+    d = np.einsum("xxyytt->xyt", T).ravel()
+    return d[np.argsort(-np.abs(d), kind="stable")[:3]]
+
+
+def cal_scaling_dimensions(T, Dcut):
+    """:613-637: eigh of the t-density matrix (its lower triangle, as
+    cupy.linalg.eigh reads it), descending, floored at 1e-16, padded to Dcut,
+    log(e0 / e) / (2 pi)."""
+    e = np.linalg.eigh(density_matrix(T, "t"), UPLO="L")[0][::-1].copy()
+    e[e < 1e-16] = 1e-16
+    e = np.pad(e, (0, Dcut - len(e)), constant_values=1e-16)
+    return np.log(np.max(e) / e) / (2 * np.pi)
+
+
+def cal_coorelation(T, Dcut, rgstep):
+    """:639-713: singular values of the three density matrices, zero-padded
+    to Dcut^2 and normalised by their largest, and ln xi = n ln 2 - ln(-1 / ln e_1)
+    with n = rgstep["Z"], ["X"], ["Y"]; as in the reference the y branch
+    tests the x leg's e_1 against 1e-200 (:700).  Returns ({leg: e}, {leg: ln xi})."""
+    ev = {}
+    for leg in "txy":
+        e = np.linalg.svd(density_matrix(T, leg), compute_uv=False)
+        e = np.pad(e, (0, max(0, Dcut * Dcut - len(e))))
+        ev[leg] = e / np.max(e)
+    lnxi = {}
+    for leg, key in (("t", "Z"), ("x", "X"), ("y", "Y")):
+        cond = ev["x"][1] if leg == "y" else ev[leg][1]
+        e1 = ev[leg][1] if cond > 1e-200 else 1e-200
+        lnxi[leg] = rgstep[key] * np.log(2) - np.log(-1 / np.log(e1))
+    return ev, lnxi
+
+
+def tensor_spectrum(T, Dcut):
+    """trg/HOTRG_3d.py:240-245 and trg/HOTRG_3d_QR.py:727-731: singular values of
+    T as the (x y z) x (x' y' z') matrix; returns (s[:2 Dcut] / max s, max s)."""
+    n = T.shape[0] * T.shape[2] * T.shape[4]
+    s = np.linalg.svd(T.transpose(0, 2, 4, 1, 3, 5).reshape(n, -1), compute_uv=False)
+    return s[:2 * Dcut] / np.max(s), np.max(s)

@@ -70,3 +70,51 @@ def test_hotrg2d_chi64_golden_against_onsager():
         tol = 1e-4 if abs(beta - 0.4406867935) < 2e-3 else 1e-6
         assert abs(v["lnZ"] - exact.onsager_lnZ(beta)) < tol, (beta, v["lnZ"], exact.onsager_lnZ(beta))
         assert abs(sum(v["lnf"]) + np.log(v["trace"]) / 2 ** 40 - v["lnZ"]) < 1e-15
+
+
+# ---------------------------------------------------------------------------
+# 3D q=3 Potts ATRG, beta = 0.55, 30 steps (trg/ATRG_3d_new.py:974-1039), the
+# benchmark's leg-swap rSVD (k = chi, p = chi, seed 1) at q = 2 and q = 20
+# ---------------------------------------------------------------------------
+GOLD_ATRG = os.path.join(HERE, "golden", "atrg3d_potts.json")
+
+
+def _atrg_runs():
+    if not os.path.exists(GOLD_ATRG):
+        return []
+    with open(GOLD_ATRG) as f:
+        return sorted({int(k.split("_")[0][3:]) for k in json.load(f)["runs"]})
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("chi", _atrg_runs() or [pytest.param(0, marks=pytest.mark.skip("no ATRG golden file"))])
+def test_atrg_potts_matches_golden(gpu, chi):
+    """Device float64 ATRG against the oracle with the same sketch: with the
+    converged range finder (q = 20) to 1e-11 relative in ln Z/V; at the
+    benchmark's q = 2, where the result depends on the sketch's gauge, within
+    ten times the oracle's own q = 2 vs q = 20 difference."""
+    from oracle import models
+    from trg_backup_amd import atrg
+    from trg_backup_amd.tensor_class.tensor_class import ATRG_Tensor
+    from trg_backup_amd.trg import ATRG_3d_new as at
+    g = _load(GOLD_ATRG)
+    ref20, ref2 = g["runs"][f"chi{chi}_q20"], g["runs"][f"chi{chi}_q2"]
+    out = {}
+    old = atrg.degeneracy_eps
+    atrg.degeneracy_eps = g["DGENERACY_EPS"]
+    try:
+        for q in (20, 2):
+            at.set_rsvd_params(k_factor=1, p=chi, q=q, seed=1)
+            t = models.to_atrg(models.potts3d_tensor(g["beta"]), chi)
+            v, T = at.ln_Z_over_V(ATRG_Tensor(t.U, t.s, t.VH, chi, 3, False, {}), chi, dict(g["steps"]))
+            out[q] = (v, T)
+    finally:
+        atrg.degeneracy_eps = old
+        at.set_rsvd_params()
+    v20, T20 = out[20]
+    v2, _ = out[2]
+    print(f"\n[atrg golden chi={chi}] q20 {v20!r} vs {ref20['lnZ']!r} ({(v20 - ref20['lnZ']) / ref20['lnZ']:.2e} rel); "
+          f"q2 {v2!r} vs {ref2['lnZ']!r} (oracle q2-q20 {ref2['lnZ'] - ref20['lnZ']:.2e})", flush=True)
+    assert list(T20.U.shape) == ref20["final_bond"]
+    assert abs(v20 - ref20["lnZ"]) < 1e-11 * abs(ref20["lnZ"])
+    assert abs(v2 - ref2["lnZ"]) <= 10 * abs(ref2["lnZ"] - ref20["lnZ"]) + 1e-12

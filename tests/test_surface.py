@@ -17,13 +17,13 @@ SURFACE = {
     "trg.HOTRG_3d": ["__tensor_legs_rearrange__", "__tensor_legs_restore__", "__contraction_projector__",
                      "new_pure_tensor", "new_impuer_tensor_2imp", "new_impuer_tensor_2to1imp",
                      "new_impuer_tensor_1imp", "__tensor_normalization__", "pure_tensor_renorm",
-                     "two_point_func_renorm", "ynearest_two_point_func_renorm"],
+                     "two_point_func_renorm", "ynearest_two_point_func_renorm", "__get_tensor_spectrum"],
     "trg.HOTRG_3d_QR": ["leg_transposition", "squeezer_fix", "squeezer_original", "squeezer_QR", "slice_legs",
                         "coarse_graining", "squeezer", "new_pure_tensor", "new_impuer_tensor_2points",
                         "new_impuer_tensor_absorb", "new_impuer_tensor_1point", "cal_X", "find_1stto3rdmax",
                         "cal_scaling_dimensions", "cal_coorelation", "normalization", "exec_rgstep_pure_tensor",
                         "pure_tensor", "cal_field_expected_value", "exec_rgstep_1point_func", "one_point_function",
-                        "exec_rgstep_2point_func", "two_point_function"],
+                        "exec_rgstep_2point_func", "two_point_function", "__get_tensor_spectrum"],
     "trg.ATRG_3d_new": ["leg_transposition", "rsvd_for_ATRG_leg_swapping", "leg_swapping", "squeezer",
                         "leg_slicing", "coarse_graining", "atrg_pure_tensor", "normalization",
                         "exec_pure_tensor_renorm", "pure_tensor", "new_impuer_tensor_2points",
@@ -40,6 +40,7 @@ SURFACE = {
     "measurement.Ising_model_3d": ["init_tensor", "init_tensor_imp", "ln_Z_over_V", "magnetization",
                                    "internal_energy"],
     "tensor_init.Potts_model": ["infinite_density"],
+    "tensor_init.ATRG_init": ["initial_tensor_for_ATRG"],
 }
 
 

@@ -30,6 +30,7 @@ ALIASES = {
     "measurement.Ising_model_3d": "trg_backup_amd.measurement.Ising_model_3d",
     "tensor_init": "trg_backup_amd.tensor_init",
     "tensor_init.Potts_model": "trg_backup_amd.tensor_init.Potts_model",
+    "tensor_init.ATRG_init": "trg_backup_amd.tensor_init.ATRG_init",
 }
 
 

@@ -163,12 +163,15 @@ def schedule(TOT_RGSTEPS):
     return order
 
 
-def renorm(T, Dcut, TOT_RGSTEPS, kind="QR", deg_eps=0.0, policy=hotrg.ALL_FP64, sync_each_step=False):
+def renorm(T, Dcut, TOT_RGSTEPS, kind="QR", deg_eps=0.0, policy=hotrg.ALL_FP64, sync_each_step=False,
+           on_step=None):
     """trg/HOTRG_3d_QR.py:747-795 for a batch T (B, x, X, y, Y, z, Z).
 
     Returns a list of groups (idx, T, lnf) covering every instance: lnf[:, n]
     = ln c_n / 2^n (c_n = |Tr T| after step n, n = 0 before the first step),
-    the values the reference stores under the keys "(X,Y,Z)"."""
+    the values the reference stores under the keys "(X,Y,Z)".  `on_step(n,
+    direction, groups)` (optional) sees the normalised groups after the
+    initial normalisation (n = 0, direction None) and after every step."""
     order = schedule(TOT_RGSTEPS)
     B = T.shape[0]
     health = ops.Health(T.device)
@@ -178,6 +181,8 @@ def renorm(T, Dcut, TOT_RGSTEPS, kind="QR", deg_eps=0.0, policy=hotrg.ALL_FP64, 
         ops.fill_(lnf, 0.0)
         hotrg.normalize_trace(T, lnf, 0)
         groups = [(list(range(B)), T, lnf)]
+        if on_step is not None:
+            on_step(0, None, groups)
         for n, direction in enumerate(order, start=1):
             nxt = []
             for idx, Tg, lg in groups:
@@ -187,6 +192,8 @@ def renorm(T, Dcut, TOT_RGSTEPS, kind="QR", deg_eps=0.0, policy=hotrg.ALL_FP64, 
                     hotrg.normalize_trace(Tn, ln, n)
                     nxt.append(([idx[i] for i in sub], Tn, ln))
             groups = nxt
+            if on_step is not None:
+                on_step(n, direction, groups)
             if sync_each_step:
                 torch.cuda.current_stream().synchronize()
     health.check("HOTRG_3d_QR solve")

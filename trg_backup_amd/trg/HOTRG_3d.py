@@ -2,15 +2,16 @@
 
 Same names and arguments as trg/HOTRG_3d.py:18-453.  Differences
 (documented): no MPI / device binding at import (:9-14); the diagnostic
-prints, density counts and spectrum SVD (:61-64, :240-245, :267-276) are not
-emitted; bonds may grow up to Dcut; untruncated steps return PM = "minus";
+prints and density counts (:61-64, :267-276) are not emitted and the flow
+does not call the spectrum diagnostic, which is a function returning its
+values (``__get_tensor_spectrum``, :240-245); bonds may grow up to Dcut; untruncated steps return PM = "minus";
 ``new_impuer_tensor_2to1imp`` restores the legs (the reference re-applies the
 rearrangement, :191-192); ``new_impuer_tensor_2imp`` uses the 3D merge (the
 reference's 2D merge on 6-leg tensors, :133-147, cannot run).
 """
 import torch
 
-from .. import hotrg, ops
+from .. import diagnostics, hotrg, ops
 from .._io import from_device, to_device
 
 PRECISION = hotrg.ALL_FP64
@@ -89,6 +90,14 @@ def __tensor_normalization__(T):
     ops.normalize_(Tb, c)
     cv = abs(float(c[0].item())) if kind == "numpy" else c[0].abs()
     return from_device(Tb[0], kind), cv
+
+
+def __get_tensor_spectrum(T, Dcut):
+    """:240-245: the leading 2 Dcut singular values of T as the (x y z) x
+    (x' y' z') matrix over the largest, and the largest (the reference prints
+    both); returns (s / s_max, s_max)."""
+    Td, _ = to_device(T)
+    return diagnostics.tensor_spectrum(Td, 2 * Dcut)
 
 
 def pure_tensor_renorm(T, Dcut, XLOOPS, YLOOPS, ZLOOPS):

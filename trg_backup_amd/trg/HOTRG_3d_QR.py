@@ -6,18 +6,18 @@ globals with defaults instead of required inputs: `select_squeezer` from
 HOTRG_SQUEEZER (default "QR"), `degeneracy_eps` from DGENERACY_EPS (default
 0), `truncate_eps` = 1e-10, and `slicing` (sys.argv[4] * 5 in the reference),
 which only splits the reference's contraction loop and does not change the
-result.  The diagnostics written on every step (cal_X, cal_coorelation,
-cal_scaling_dimensions, prints and timings) are not emitted.  The squeezers
-take the pure-tensor form (T0 = T1 = T2 = T3); of the impurity flows the
-one-point (:520-557, :798-908) and two-point (:428-518, :910-1033) flows
-are on the device; the diagnostics are not.
+result.  The per-step diagnostics (cal_X, cal_coorelation,
+cal_scaling_dimensions) run on the device and write their files when
+OUTPUT_DIR is set; prints and timings are not emitted.  The squeezers take
+the pure-tensor form (T0 = T1 = T2 = T3); the one-point (:520-557, :798-908)
+and two-point (:428-518, :910-1033) impurity flows are on the device.
 """
 import math
 import os
 
 import torch
 
-from .. import hotrg, hotrg_qr, ops
+from .. import diagnostics, hotrg, hotrg_qr, ops
 from .._io import from_device, to_device
 from .._lib import TrgError
 
@@ -149,9 +149,13 @@ def exec_rgstep_pure_tensor(T, Dcut, direction, rgstep):
 def pure_tensor(T, Dcut, TOT_RGSTEPS):
     """trg/HOTRG_3d_QR.py:747-795: the whole flow on the device (one instance,
     no host round trip per step); fills T['factor'] with the reference's
-    "(X,Y,Z)" keys and returns T."""
+    "(X,Y,Z)" keys and returns T.  With OUTPUT_DIR set, the X ratios,
+    correlation lengths and scaling dimensions of the initial and of every
+    coarse-grained tensor go to the reference's files (:757-759, :781-783)."""
     Td, kind = to_device(T["Tensor"])
-    groups = hotrg_qr.renorm(Td[None], Dcut, TOT_RGSTEPS, select_squeezer, degeneracy_eps, PRECISION)
+    rgstep.update({"X": 0, "Y": 0, "Z": 0})
+    hook = _step_diagnostics(Dcut) if os.environ.get("OUTPUT_DIR") else None
+    groups = hotrg_qr.renorm(Td[None], Dcut, TOT_RGSTEPS, select_squeezer, degeneracy_eps, PRECISION, on_step=hook)
     (_, Tn, lnf), = groups
     for key, v in zip(hotrg_qr.factor_keys(TOT_RGSTEPS), lnf[0].tolist()):
         T["factor"][key] = v
@@ -251,17 +255,123 @@ def one_point_function(T, T0, Dcut, TOT_RGSTEPS):
     return T, T0
 
 
-def _not_on_device(name, lines):
-    def f(*args, **kwargs):
-        raise NotImplementedError(f"HOTRG_3d_QR.{name} (trg/HOTRG_3d_QR.py:{lines}) is not on the device")
-    f.__name__ = name
-    return f
+# ---------------------------------------------------------------------------
+# diagnostics (trg/HOTRG_3d_QR.py:559-713) and their per-step files
+# ---------------------------------------------------------------------------
+# step counters of the running flow (the reference's global, :747-750); the
+# diagnostics name their files and ln xi by them
+rgstep = {"X": 0, "Y": 0, "Z": 0}
 
 
-cal_X = _not_on_device("cal_X", "559-583")
-find_1stto3rdmax = _not_on_device("find_1stto3rdmax", "585-611")
-cal_scaling_dimensions = _not_on_device("cal_scaling_dimensions", "613-637")
-cal_coorelation = _not_on_device("cal_coorelation", "639-713")
+def _mode():
+    return "w" if sum(rgstep.values()) == 0 else "a"
+
+
+def _out_dir(save):
+    d = os.environ.get("OUTPUT_DIR")
+    return d if save and d else None
+
+
+def cal_X(Tensor, save=False):
+    """:559-583: X_t, X_x, X_y = (Tr T)^2 / TT with TT the trace of two tensors
+    stacked along the leg, on the device; with ``save`` one line
+    "Xt Xx Xy" goes to OUTPUT_DIR/X.dat (new file at step 0)."""
+    Td, _ = to_device(Tensor["Tensor"])
+    Xt, Xx, Xy = diagnostics.x_ratios(Td)
+    d = _out_dir(save)
+    if d:
+        with open(os.path.join(d, "X.dat"), _mode()) as f:
+            f.write(f"{Xt:.12e} {Xx:.12e} {Xy:.12e}\n")
+    return Xt, Xx, Xy
+
+
+def find_1stto3rdmax(Tensor, save=False):
+    """:585-611: the three diagonal entries T[x,x,y,y,t,t] of largest |value|
+    (the diagonal is gathered on the device, the three picked on the host);
+    with ``save`` one line goes to OUTPUT_DIR/max_1to3-th_values.dat.  The
+    reference returns nothing; this returns the three values."""
+    import numpy as np
+    Td, _ = to_device(Tensor["Tensor"])
+    dg = diagnostics.diagonal(Td).cpu().numpy()
+    top = np.pad(dg[np.argsort(-np.abs(dg), kind="stable")[:3]], (0, max(0, 3 - dg.size)))
+    d = _out_dir(save)
+    if d:
+        with open(os.path.join(d, "max_1to3-th_values.dat"), _mode()) as f:
+            f.write(f"{top[0]:40.12e} {top[1]:40.12e} {top[2]:40.12e}\n")
+    return top
+
+
+def cal_scaling_dimensions(Tensor, Dcut, save=False):
+    """:613-637: eigenvalues of the t-density matrix "xxyytT->Tt" (device
+    Jacobi on its lower triangle), descending, floored at 1e-16, padded to
+    Dcut with 1e-16, log(e0 / e) / (2 pi); with ``save`` one line goes to
+    OUTPUT_DIR/scaling_dimensions.dat.  Returns the scaling dimensions."""
+    import numpy as np
+    Td, _ = to_device(Tensor["Tensor"])
+    e = diagnostics.eigvals_lower(diagnostics.density_matrix(Td, "t")).cpu().numpy()[::-1].copy()
+    e[e < 1e-16] = 1e-16
+    e = np.pad(e, (0, max(0, Dcut - len(e))), constant_values=1e-16)
+    sd = np.log(np.max(e) / e) / (2 * np.pi)
+    d = _out_dir(save)
+    if d:
+        with open(os.path.join(d, "scaling_dimensions.dat"), _mode()) as f:
+            f.write(" ".join(f"{v:.12e}" for v in sd) + "\n")
+    return sd
+
+
+def cal_coorelation(Tensor, Dcut, save=False):
+    """:639-713: singular values (device Jacobi) of the t, x and y density
+    matrices, zero-padded to Dcut^2 and normalised by their largest, and
+    ln xi = n ln 2 - ln(-1 / ln e_1) with n = rgstep["Z"], ["X"], ["Y"]; the
+    y branch tests the x leg's e_1 against 1e-200, as the reference does
+    (:700).  With ``save`` the files lnξt.dat, lnξx.dat, lnξy.dat and
+    density_matrix_t/DentMatT_lx*_ly*_lt*.dat are written under OUTPUT_DIR.
+    Returns ({leg: e}, {leg: ln xi}) (the reference prints them)."""
+    import numpy as np
+    Td, _ = to_device(Tensor["Tensor"])
+    ev, vmax = {}, {}
+    for leg in "txy":
+        e = diagnostics.singular_values(diagnostics.density_matrix(Td, leg)).cpu().numpy()
+        e = np.pad(e, (0, max(0, Dcut * Dcut - len(e))))
+        vmax[leg] = float(np.max(e))
+        ev[leg] = e / vmax[leg]
+    lnxi = {}
+    for leg, key in (("t", "Z"), ("x", "X"), ("y", "Y")):
+        cond = ev["x"][1] if leg == "y" else ev[leg][1]
+        e1 = ev[leg][1] if cond > 1e-200 else 1e-200
+        lnxi[leg] = rgstep[key] * math.log(2) - math.log(-1 / math.log(e1))
+    d = _out_dir(save)
+    if d:
+        mode = _mode()
+        for leg in "txy":
+            with open(os.path.join(d, f"lnξ{leg}.dat"), mode) as f:
+                f.write(f"{lnxi[leg]:.12e}\n")
+        dm = os.path.join(d, "density_matrix_t")
+        os.makedirs(dm, exist_ok=True)
+        with open(os.path.join(dm, f"DentMatT_lx{rgstep['X']}_ly{rgstep['Y']}_lt{rgstep['Z']}.dat"), mode) as f:
+            f.write(f"λ1={vmax['t']:.12e}\n")
+            f.writelines(f"{v:.12e}\n" for v in ev["t"])
+    return ev, lnxi
+
+
+def __get_tensor_spectrum(T, Dcut):
+    """:726-730: the leading 2 Dcut singular values of T as the (x y z) x
+    (x' y' z') matrix over the largest (the reference prints them); returns
+    (s / s_max, s_max)."""
+    Td, _ = to_device(T)
+    return diagnostics.tensor_spectrum(Td, 2 * Dcut)
+
+
+def _step_diagnostics(Dcut):
+    """The per-step calls of :757-759 and :781-783 as a renorm hook."""
+    def hook(n, direction, groups):
+        if direction is not None:
+            rgstep[direction] += 1
+        T = {"Tensor": groups[0][1][0]}
+        cal_X(T, save=True)
+        cal_coorelation(T, Dcut, save=True)
+        cal_scaling_dimensions(T, Dcut, save=True)
+    return hook
 
 
 # ---------------------------------------------------------------------------

@@ -36,12 +36,28 @@ def hotrg3d_betas(nb=128):
     return np.random.default_rng(0).uniform(0.20, 0.24, 128)[:nb].tolist()
 
 
-def hotrg3d_policy(fp64_steps=4, merge="bf16x6"):
+def hotrg3d_policy(fp64_steps=4, merge="bf16x6", split_merge_from=None, bf16_merge_from=None,
+                   bf16_env_from=None):
     """float64 through step 4 (the first full chi^11 step), then float32
     storage with three-piece split-bf16 GEMMs (fp32 accuracy), or float32 MFMA
-    with merge="f32"."""
+    with merge="f32"; optionally split-bf16 merges after `split_merge_from`,
+    bf16 merges after `bf16_merge_from` and bf16 environments after
+    `bf16_env_from` (None or a value below 0: never)."""
+    opt = lambda v: None if v is None or v < 0 else v
     split6 = fp64_steps if merge == "bf16x6" else None
-    return hotrg.Policy(fp64_steps=fp64_steps, low=hotrg.FP32, split6_merge_from=split6, split6_env_from=split6)
+    return hotrg.Policy(fp64_steps=fp64_steps, low=hotrg.FP32, split6_merge_from=split6, split6_env_from=split6,
+                        split_merge_from=opt(split_merge_from), bf16_merge_from=opt(bf16_merge_from),
+                        bf16_env_from=opt(bf16_env_from))
+
+
+def tiers(policy, nsteps, first=1):
+    """[(first step, Precision)] of each run of equal precision in steps first..nsteps."""
+    out = []
+    for n in range(first, nsteps + 1):
+        p = policy(n)
+        if not out or (p.store, p.env, p.merge) != (out[-1][1].store, out[-1][1].env, out[-1][1].merge):
+            out.append((n, p))
+    return out
 
 
 def schedule2d(nsteps):
