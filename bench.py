@@ -62,7 +62,7 @@ def parse():
     ap.add_argument("--chi", type=int, default=None, help="hotrg2d 64, hotrg3d 16, atrg3d 64")
     ap.add_argument("--nbeta", type=int, default=None, help="hotrg2d 32, hotrg3d 32 (of the 128-value draw)")
     ap.add_argument("--fp64-steps", type=int, default=None,
-                    help="hotrg2d 6, hotrg3d 4, atrg3d 16, hotrg3d_qr all (its oblique projectors amplify "
+                    help="hotrg2d 6, hotrg3d 4, atrg3d 14, hotrg3d_qr all (its oblique projectors amplify "
                          "float32 rounding: f32 after step 4 gives 5e-8)")
     ap.add_argument("--atrg-q", type=int, default=2)
     ap.add_argument("--squeezer", default="QR", choices=["original", "QR", "fix"], help="hotrg3d_qr squeezer")
@@ -633,7 +633,7 @@ def main():
     args.steps_requested = args.steps
     args.steps = args.steps or {"hotrg2d": 40, "hotrg3d": 24, "hotrg3d_qr": 6, "atrg3d": 30}.get(wl, 40)
     args.fp64_steps = (args.fp64_steps if args.fp64_steps is not None
-                       else {"hotrg2d": 6, "hotrg3d": 4, "hotrg3d_qr": 10 ** 6, "atrg3d": 16}.get(wl, 0))
+                       else {"hotrg2d": 6, "hotrg3d": 4, "hotrg3d_qr": 10 ** 6, "atrg3d": 14}.get(wl, 0))
     if args.gemm_sweep:
         return gemm_sweep(args)
     if args.traffic_probe:
