@@ -374,11 +374,12 @@ def merge3d(T1, T2, U1a, U1b, U2a, U2b, compute=None):
     """ijklez,mnopzf,ima,jnb,koc,lpd->abcdef (trg/HOTRG_3d.py:107-116), batched.
 
     X1[(j,k,l,e,z),(m,a)] = sum_i T1 U1a                      2 chi^8
-    X2[(o?)]: X1 . U2a over k                                  2 chi^9
+    X2[(j,l,e,z,m,a),(o,c)] = sum_k X1 U2a                     2 chi^9
     X3[(j,l,e,a,c),(n,p,f)] = sum_(z,m,o) X2 T2               2 chi^11
     X4 = X3 . U1b over (j,n) ; out = X4 . U2b over (l,p)      2 chi^9 + 2 chi^8
-    The first index i of T1 ... the output leg a is chunked to bound the chi^8
-    intermediates by MERGE_BUDGET.
+    The output leg a is chunked (INSTANCE_BUDGET, a function of the shape
+    only) and the batch split into sub-batches (MERGE_BUDGET) to bound the
+    chi^8 intermediates.
     """
     B = T1.shape[0]
     Dx, _, Dy, _, Dz, _ = T1.shape[1:]
@@ -396,33 +397,59 @@ def merge3d(T1, T2, U1a, U1b, U2a, U2b, compute=None):
             ops.copy_into(res[b0:b0 + outs[i].shape[0]], outs[i])
         return res
     out = torch.empty((B, ca, cb, cc_, cd, Dz, Dz), dtype=T1.dtype, device=T1.device)
-    # T2 as [(z,m,o),(n,p,f)]: permute legs (m,n,o,p,z,f) -> (z,m,o,n,p,f).  The bf16 MFMA
-    # wants K-contiguous operands: there T2 is stored [(n,p,f),(z,m,o)] and read transposed
-    # (same product bits in either layout)
-    if compute in SPLIT_COMPUTE:
-        T2p = permute(T2, (0, 2, 4, 6, 5, 1, 3)).view(B, Dx * Dy * Dz, Dz * Dx * Dy).transpose(1, 2)
-    else:
-        T2p = permute(T2, (0, 5, 1, 3, 2, 4, 6)).view(B, Dz * Dx * Dy, Dx * Dy * Dz)
     T1m = T1.reshape(B, Dx, Dx * Dy * Dy * Dz * Dz)                     # [i,(j,k,l,e,z)]
+    kinner = compute in SPLIT_COMPUTE
+    if kinner:
+        # The bf16 MFMA wants K-contiguous operands.  The second and third GEMMs are
+        # computed transposed and write their products through two-level output
+        # indices straight into the layout the next GEMM reads, so neither chi^8
+        # intermediate is permuted: X2 lands as [l,e,a,c,j,(z,m,o)] and X3 as
+        # [(l,e,a,c,p,f),(j,n)].  T2 is stored [p,f,n,(z,m,o)] (legs m,n,o,p,z,f)
+        # and U2a as [(c,o),k].
+        T2p = permute(T2, (0, 4, 6, 2, 5, 1, 3)).view(B, Dy * Dz * Dx, Dz * Dx * Dy)    # [(p,f,n),(z,m,o)]
+        U2aT = permute(U2a, (0, 3, 2, 1)).view(B, cc_ * Dy, Dy)                         # [(c,o),k]
+    else:
+        # T2 as [(z,m,o),(n,p,f)]: permute legs (m,n,o,p,z,f) -> (z,m,o,n,p,f)
+        T2p = permute(T2, (0, 5, 1, 3, 2, 4, 6)).view(B, Dz * Dx * Dy, Dx * Dy * Dz)
+    zmo = Dz * Dx * Dy
     for a0 in range(0, ca, na):
         a1 = min(ca, a0 + na)
         nca = a1 - a0
         # X1[(j,k,l,e,z),(m,a)] = sum_i T1[i,(jklez)] U1a[i,(m,a)]
         U1s = U1a[:, :, :, a0:a1]                                       # [i,m,a]
         X1 = gemm(T1m.transpose(1, 2), ops.contiguous(U1s).view(B, Dx, Dx * nca), compute=compute)
-        # X1 legs (j,k,l,e,z,m,a) -> X1p [(j,l,e,z,m,a),(k)]
-        X1p = permute(X1.view(B, Dx, Dy, Dy, Dz, Dz, Dx, nca), (0, 1, 3, 4, 5, 6, 7, 2))
-        X2 = gemm(X1p.view(B, -1, Dy), U2a.reshape(B, Dy, Dy * cc_), compute=compute)  # [(j,l,e,z,m,a),(o,c)]
-        # X2 legs (j,l,e,z,m,a,o,c) -> [(j,l,e,a,c),(z,m,o)]
-        X2p = permute(X2.view(B, Dx, Dy, Dz, Dz, Dx, nca, Dy, cc_), (0, 1, 2, 3, 6, 8, 4, 5, 7))
-        del X1, X1p, X2
-        X3 = gemm(X2p.view(B, Dx * Dy * Dz * nca * cc_, Dz * Dx * Dy), T2p, compute=compute,
-                  tag="merge3d_x3")                                        # [(j,l,e,a,c),(n,p,f)]
-        del X2p
-        # X3 legs (j,l,e,a,c,n,p,f) -> [(l,e,a,c,p,f),(j,n)]
-        X3p = permute(X3.view(B, Dx, Dy, Dz, nca, cc_, Dx, Dy, Dz), (0, 2, 3, 4, 5, 7, 8, 1, 6))
-        del X3
-        X4 = gemm(X3p.view(B, -1, Dx * Dx), U1b.reshape(B, Dx * Dx, cb), compute=compute)  # [(l,e,a,c,p,f),b]
+        if kinner:
+            # X1 legs (j,k,l,e,z,m,a) -> X1p [(l,e,a,j,z,m),(k)]
+            X1p = permute(X1.view(B, Dx, Dy, Dy, Dz, Dz, Dx, nca), (0, 3, 4, 7, 1, 5, 6, 2))
+            del X1
+            lea, jzm = Dy * Dz * nca, Dx * Dz * Dx
+            # X2^T[(c,o),(l,e,a,j,z,m)] = U2a^T . X1p^T, written as [l,e,a,c,j,z,m,o]:
+            # rows (c | o) at strides (Dx zmo | 1), columns (l,e,a | j,z,m) at (cc Dx zmo | Dy)
+            X2p = torch.empty(B * lea * cc_ * Dx * zmo, dtype=T1.dtype, device=T1.device)
+            gemm(U2aT, X1p.view(B, lea * jzm, Dy).transpose(1, 2), X2p, compute=compute,
+                 out_layout=([(cc_, Dx * zmo), (Dy, 1)], [(lea, cc_ * Dx * zmo), (jzm, Dy)]))
+            del X1p
+            # X3^T[(p,f,n),(l,e,a,c,j)] = T2p . X2p^T, written as [(l,e,a,c,p,f),(j,n)]:
+            # rows (p,f | n) at strides (Dx^2 | 1), columns (l,e,a,c | j) at (Dy Dz Dx^2 | Dx)
+            X3p = torch.empty(B * lea * cc_ * Dy * Dz * Dx * Dx, dtype=T1.dtype, device=T1.device)
+            gemm(T2p, X2p.view(B, lea * cc_ * Dx, zmo).transpose(1, 2), X3p, compute=compute, tag="merge3d_x3",
+                 out_layout=([(Dy * Dz, Dx * Dx), (Dx, 1)], [(lea * cc_, Dy * Dz * Dx * Dx), (Dx, Dx)]))
+            del X2p
+            X3p = X3p.view(B, Dy, Dz, nca, cc_, Dy, Dz, Dx, Dx)         # [l,e,a,c,p,f,j,n]
+        else:
+            # X1 legs (j,k,l,e,z,m,a) -> X1p [(j,l,e,z,m,a),(k)]
+            X1p = permute(X1.view(B, Dx, Dy, Dy, Dz, Dz, Dx, nca), (0, 1, 3, 4, 5, 6, 7, 2))
+            X2 = gemm(X1p.view(B, -1, Dy), U2a.reshape(B, Dy, Dy * cc_), compute=compute)  # [(j,l,e,z,m,a),(o,c)]
+            # X2 legs (j,l,e,z,m,a,o,c) -> [(j,l,e,a,c),(z,m,o)]
+            X2p = permute(X2.view(B, Dx, Dy, Dz, Dz, Dx, nca, Dy, cc_), (0, 1, 2, 3, 6, 8, 4, 5, 7))
+            del X1, X1p, X2
+            X3 = gemm(X2p.view(B, Dx * Dy * Dz * nca * cc_, zmo), T2p, compute=compute,
+                      tag="merge3d_x3")                                    # [(j,l,e,a,c),(n,p,f)]
+            del X2p
+            # X3 legs (j,l,e,a,c,n,p,f) -> [(l,e,a,c,p,f),(j,n)]
+            X3p = permute(X3.view(B, Dx, Dy, Dz, nca, cc_, Dx, Dy, Dz), (0, 2, 3, 4, 5, 7, 8, 1, 6))
+            del X3
+        X4 = gemm(X3p.reshape(B, -1, Dx * Dx), U1b.reshape(B, Dx * Dx, cb), compute=compute)  # [(l,e,a,c,p,f),b]
         del X3p
         # X4 legs (l,e,a,c,p,f,b) -> [(e,a,c,f,b),(l,p)]
         X4p = permute(X4.view(B, Dy, Dz, nca, cc_, Dy, Dz, cb), (0, 2, 3, 4, 6, 7, 1, 5))
