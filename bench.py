@@ -94,6 +94,8 @@ def parse():
     ap.add_argument("--serve-n", type=int, default=24)
     ap.add_argument("--serve-chis", default="16,24,32", help="bond dimensions of the request stream")
     ap.add_argument("--serve-steps", default="20,30", help="RG-step range of the request stream")
+    ap.add_argument("--serve-check", type=int, default=3,
+                    help="ATRG requests of the stream re-solved alone for the accuracy check (smallest chi first)")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="hotrg3d: eager steady steps instead of HIP graphs")
     ap.add_argument("--progress", action="store_true",
@@ -462,26 +464,44 @@ def serve_leg(args):
     """The request stream through serve.Server: Poisson arrivals (rate 50/s, seed 2), model uniform
     over {2D Ising HOTRG, 3D Ising ATRG, 3D q=3 Potts ATRG}, chi uniform over --serve-chis, beta
     uniform within 10% of the model's critical coupling, steps uniform in --serve-steps.  2D flows
-    run float64 for 10 steps then float32; ATRG flows run float64 (rSVD k = 2 chi, q = 2)."""
+    run float64 for 10 steps then float32; ATRG flows run float64 (rSVD k = 2 chi, q = 2), flows of equal
+    bond shapes stepped in batched groups."""
     import torch
 
-    from trg_backup_amd import serve
+    from trg_backup_amd import atrg, serve
     chis = tuple(int(c) for c in args.serve_chis.split(","))
     lo, hi = (int(c) for c in args.serve_steps.split(","))
     reqs = serve.poisson_stream(args.serve_n, 50.0, 2, chis=chis, steps=(lo, hi))
     srv = serve.Server()
     t1 = time.perf_counter()
-    done = srv.run(reqs, realtime=True)
+    done = srv.run(reqs, realtime=True, log=lambda m: print("[serve]", m, file=sys.stderr, flush=True))
     ts = time.perf_counter() - t1
     summ = serve.summarize(done, ts)
+    # accuracy of every model: the 2D flows (float32 after step 10) against float64 solves of the
+    # same requests; the ATRG flows (float64, stepped in batched groups) against a sample of the same
+    # requests solved alone, one flow at a time (the per-instance float64 path)
     twod = [f.req for f in done if f.req.model == "ising2d"]
-    ref = {f.req.rid: f.result for f in serve.Server(fp64_steps=10 ** 9).run(twod, realtime=False)} if twod else {}
-    err = max([abs(f.result - ref[f.req.rid]) for f in done if f.req.rid in ref] or [0.0])
+    slog = lambda m: print("[serve]", m, file=sys.stderr, flush=True)
+    slog(f"stream served in {ts:.1f} s; float64 re-solves of the {len(twod)} 2D requests")
+    ref = ({f.req.rid: f.result for f in serve.Server(fp64_steps=10 ** 9).run(twod, realtime=False, log=slog)}
+           if twod else {})
+    sample = [f.req for f in done if f.req.model != "ising2d" and f.error is None]
+    sample = sorted(sample, key=lambda r: (r.chi, r.rid))[:args.serve_check]
+    for r in sample:
+        ref[r.rid] = serve.Server(streams=1).run([r], realtime=False)[0].result
+        slog(f"rid={r.rid} {r.model} chi={r.chi} re-solved alone")
+    errs = {}
+    for f in done:
+        if f.req.rid in ref and f.error is None:
+            errs[f.req.model] = max(errs.get(f.req.model, 0.0), abs(f.result - ref[f.req.rid]))
+    checked = {m: sum(1 for f in done if f.req.rid in ref and f.req.model == m) for m in errs}
     by = {}
     for f in done:
         by.setdefault(f.req.model, []).append(f.finished_at - f.req.arrival)
     torch.cuda.empty_cache()
-    return dict(summ, worst_lnZ_err_vs_fp64_2d=err, rejected=len(srv.rejected), wall_s=ts,
+    return dict(summ, worst_lnZ_err_vs_fp64=max(errs.values() or [0.0]), lnZ_err_by_model=errs,
+                lnZ_checked_by_model=checked, errors=sum(1 for f in done if f.error),
+                atrg_rank_splits=atrg.SPLITS, rejected=len(srv.rejected), wall_s=ts,
                 max_hbm_admitted_gb=srv.max_used / 2 ** 30,
                 latency_by_model={m: {"n": len(v), "p50_s": float(sorted(v)[len(v) // 2]), "max_s": max(v)}
                                   for m, v in by.items()},

@@ -398,3 +398,66 @@ def test_atrg_init_and_inte_rsvds_exact_when_sketch_covers_rank(gpu):
     assert np.allclose(s2[:r2], s2_ref[:r2], rtol=1e-9, atol=1e-9 * s2_ref[0])
     M2r = np.einsum("ijkl,l,labc->ijkabc", u2[..., :r2], s2[:r2], vh2[:r2]).reshape(27, 27)
     assert np.abs(M2r - M2).max() < 1e-9 * np.abs(M2).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("betas", [(0.53, 0.55, 0.57), (0.01, 0.55, 0.30)])
+def test_atrg_batched_steps_equal_alone(gpu, betas):
+    """atrg.atrg_pure_tensor_batch (trg/ATRG_3d_new.py:441-482 for several
+    instances per launch) gives every instance, bit for bit, the tensors it
+    gets stepped alone: every kernel is batch-invariant, and where the
+    instances' rank decisions part (the second set runs with a 1e-4 tail cut,
+    under which the weakly coupled instance keeps fewer values) the step is re-run per group of equal
+    decisions.  normalize_batch equals normalize per instance."""
+    from trg_backup_amd import atrg
+    from trg_backup_amd import models as dm
+    chi, steps = 10, 7
+    saved = (dict(atrg.RSVD), atrg.degeneracy_eps, atrg.truncate_eps)
+    atrg.set_rsvd_params(k_factor=2, p=0, q=2, seed=1)
+    atrg.degeneracy_eps = 1e-8
+    if 0.01 in betas:
+        # a coarse tail cut makes the kept ranks depend on the coupling
+        atrg.truncate_eps = 1e-4
+    splits0 = atrg.SPLITS
+    try:
+        alone, batched = [], []
+        for b in betas:
+            U, s, VH = dm.atrg_initial("potts3d", b)
+            s, _ = atrg.normalize(U, s, VH)
+            alone.append((U, s, VH))
+            batched.append((U, s, VH))
+        for n in range(1, steps + 1):
+            d = "TXY"[(n - 1) % 3]
+            alone = [atrg.atrg_pure_tensor(U, s, VH, chi, d, "f64") for U, s, VH in alone]
+            groups = {}
+            for i, (U, s, VH) in enumerate(batched):
+                groups.setdefault((tuple(U.shape), tuple(VH.shape)), []).append(i)
+            nxt = [None] * len(batched)
+            for idx in groups.values():
+                res = atrg.atrg_pure_tensor_batch(atrg._stack([batched[i][0] for i in idx]),
+                                                  atrg._stack([batched[i][1] for i in idx]),
+                                                  atrg._stack([batched[i][2] for i in idx]), chi, d, "f64")
+                for j, i in enumerate(idx):
+                    nxt[i] = res[j]
+            batched = nxt
+            for i in range(len(betas)):
+                for x, y in zip(alone[i], batched[i]):
+                    assert x.shape == y.shape and torch.equal(x, y), (betas[i], n)
+            # normalisation: batched over the instances of equal shape, each against its own
+            idx = [i for i in range(len(betas)) if batched[i][0].shape == batched[0][0].shape
+                   and batched[i][2].shape == batched[0][2].shape and batched[i][1].shape == batched[0][1].shape]
+            sb, cb = atrg.normalize_batch(atrg._stack([batched[i][0] for i in idx]),
+                                          atrg._stack([batched[i][1] for i in idx]),
+                                          atrg._stack([batched[i][2] for i in idx]))
+            for j, i in enumerate(idx):
+                s1, c1 = atrg.normalize(*alone[i])
+                assert torch.equal(sb[j], s1) and torch.equal(cb[j:j + 1], c1)
+            alone = [(U, atrg.normalize(U, s, VH)[0], VH) for U, s, VH in alone]
+            batched = [(U, atrg.normalize(U, s, VH)[0], VH) for U, s, VH in batched]
+        print(f"\n[atrg batched {betas}] {steps} steps bitwise equal alone; rank splits {atrg.SPLITS - splits0}",
+              flush=True)
+        if 0.01 in betas:
+            assert atrg.SPLITS > splits0
+    finally:
+        atrg.RSVD.update(saved[0])
+        atrg.degeneracy_eps, atrg.truncate_eps = saved[1], saved[2]

@@ -16,7 +16,9 @@ def _reqs():
             Request(3, "ising2d", 8, 0.41, 10, 0.02),
             Request(4, "ising3d", 6, 0.22, 6, 0.0),
             Request(5, "potts3d", 6, 0.55, 6, 0.03),
-            Request(6, "ising2d", 16, 0.44, 12, 0.05)]
+            Request(6, "ising2d", 16, 0.44, 12, 0.05),
+            Request(7, "potts3d", 6, 0.56, 6, 0.0),
+            Request(8, "potts3d", 6, 0.55, 6, 0.04)]
 
 
 def test_each_result_is_the_same_alone_and_in_the_mix(gpu):
@@ -34,6 +36,7 @@ def test_each_result_is_the_same_alone_and_in_the_mix(gpu):
         print(f"\n[serve] rid={rid} alone={alone[rid]!r} mixed={mixed[rid]!r}", flush=True)
         assert mixed[rid] == alone[rid], (rid, mixed[rid], alone[rid])
     assert mixed[0] == mixed[6]      # identical requests, different arrival and batch
+    assert mixed[5] == mixed[8]      # the same for ATRG flows stepped in batched groups
 
 
 def test_served_2d_result_matches_oracle(gpu):

@@ -23,7 +23,7 @@ import math
 import torch
 
 from . import linalg, ops
-from ._lib import TrgError
+from ._lib import TrgError, call
 from .contract import contract
 
 degeneracy_eps = 0.0
@@ -408,6 +408,295 @@ def atrg_pairs_step(U, s, VH, pairs, Dcut, direction, prec="f64"):
     U2, s2, VH2 = coarse_graining(A, UM, sM, VMH, D, Px, PX, Py, PY)
     U2, VH2 = leg_transposition(U2, VH2, "restore", direction)
     return (U2, s2, VH2), out
+
+
+# ---------------------------------------------------------------------------
+# batched pure steps: several instances of equal bond shapes per launch
+# ---------------------------------------------------------------------------
+SPLITS = 0          # batched steps re-run per group of equal rank decisions (a counter)
+
+
+class RankSplit(Exception):
+    """Raised inside a batched step when the instances' rank decisions (or the
+    range finder's breakdown flags) disagree; `keys` holds one value per
+    instance, and the caller re-runs the step for each group of equal keys."""
+
+    def __init__(self, keys):
+        super().__init__(f"rank decisions differ across the batch: {keys}")
+        self.keys = keys
+
+
+def _uniform(keys):
+    keys = list(keys)
+    if len(set(keys)) > 1:
+        raise RankSplit(keys)
+    return keys[0]
+
+
+def _truncate_b(s, k, deg_eps, trunc_eps):
+    """linalg.truncate for every row of s (B, n): one launch of the rank rule
+    per instance, one host read for all of them."""
+    Bn, n = s.shape
+    if deg_eps == 0 and trunc_eps == 0:
+        return [int(k)] * Bn
+    if k > n:
+        return [n] * Bn
+    sc = ops.contiguous(_f64(s))
+    chi = torch.empty(Bn, dtype=torch.int32, device=s.device)
+    err = torch.empty(Bn, dtype=torch.float64, device=s.device)
+    for b in range(Bn):
+        call("trg_truncate_rank", n, ops.ptr(sc[b]), int(k), float(deg_eps), float(trunc_eps), ops.ptr(chi[b:b + 1]),
+             ops.ptr(err[b:b + 1]), ops.stream_ptr())
+    return [int(v) for v in chi.cpu().tolist()]
+
+
+def _count_above_b(s, eps):
+    """linalg.count_above for every row of s (B, n), one host read."""
+    Bn, n = s.shape
+    flags = ops.contiguous(_f64(s))
+    c = ops.maxabs(flags, batch=Bn)
+    ops.normalize_(flags, c, batch=Bn)
+    vals = flags.cpu()
+    return [int((vals[b] > eps).sum().item()) for b in range(Bn)]
+
+
+def _stack(ts):
+    """Stack equal-shape device tensors into one (B, ...) buffer with libtrg copies."""
+    out = torch.empty((len(ts),) + tuple(ts[0].shape), dtype=ts[0].dtype, device=ts[0].device)
+    for b, t in enumerate(ts):
+        ops.copy_into(out[b], t)
+    return out
+
+
+def _leg_transposition_b(U, VH, do_what, direction):
+    if direction in ("t", "T"):
+        return U, VH
+    fwd = do_what == "transpose"
+    if direction in ("x", "X"):
+        ua, va = ((1, 2, 0, 3), (0, 2, 3, 1)) if fwd else ((2, 0, 1, 3), (0, 3, 1, 2))
+    else:
+        ua, va = ((2, 0, 1, 3), (0, 3, 1, 2)) if fwd else ((1, 2, 0, 3), (0, 2, 3, 1))
+    sh = lambda p: (0,) + tuple(i + 1 for i in p)
+    return ops.permute(U, sh(ua)), ops.permute(VH, sh(va))
+
+
+def _rsvd_leg_swap_b(Bt, Ct, chi_k, p, q, eps, seed, keep):
+    """rsvd_for_ATRG_leg_swapping for a batch: Bt (B, b, i, t), Ct (B, t, j, g)."""
+    Bn, nb, ni, nt = Bt.shape
+    _, _, nj, ng = Ct.shape
+    l = chi_k + p
+    rows_i, rows_j = ni * ng, nj * nb
+    ls = min(l, rows_i, rows_j)
+    dt = Bt.dtype
+
+    def apply_M(Y):
+        W = contract("Zbit,Zjbl->Zitjl", Bt, Y)
+        return contract("Zitjl,Ztjg->Zigl", W, Ct)
+
+    def apply_MT(Q):
+        W = contract("Ztjg,Zigl->Ztjil", Ct, Q)
+        return contract("Ztjil,Zbit->Zjbl", W, Bt)
+
+    if ls >= min(rows_i, rows_j) and min(rows_i, rows_j) > linalg.SMALL_MAX:
+        ls = linalg.SMALL_MAX
+    elif ls > linalg.SMALL_MAX:
+        ls = linalg.SMALL_MAX
+    if ls >= min(rows_i, rows_j):
+        M = contract("Zbit,Ztjg->Zigjb", Bt, Ct).reshape(Bn, rows_i, rows_j)
+        UM, sM, VMH = linalg.svd_thin(M)
+    else:
+        def sketch(ls, perturb):
+            infos = []
+
+            def orth(X):
+                Q, info = linalg.orthonormalize(X, perturb=perturb, return_info=True)
+                infos.append(info)
+                return Q
+            Y = ops.gaussian((nj, nb, l), seed=seed, device=Bt.device, batch=Bn, stream_stride=0)
+            Y = ops.contiguous(Y.view(Bn, nj, nb, l)[..., :ls], dt)
+            Q = orth(apply_M(Y).reshape(Bn, rows_i, ls))
+            for _ in range(q):
+                Z = apply_MT(ops.cast(Q.view(Bn, ni, ng, ls), dt))
+                Q2 = orth(_f64(Z).reshape(Bn, rows_j, ls))
+                Y = apply_M(ops.cast(Q2.view(Bn, nj, nb, ls), dt))
+                Q = orth(_f64(Y).reshape(Bn, rows_i, ls))
+            XT = _f64(apply_MT(ops.cast(Q.view(Bn, ni, ng, ls), dt))).reshape(Bn, rows_j, ls)
+            v, s, uh = linalg.svd_thin(XT)
+            bad = [int(x) != 0 for x in torch.stack(infos).max(dim=0).values.cpu().tolist()]
+            return ops.gemm(Q, ops.contiguous(uh.transpose(1, 2))), s, ops.contiguous(v.transpose(1, 2)), bad
+
+        floor = max(1, min(ls, keep if keep else ls))
+        UM, sM, VMH, bad = sketch(ls, linalg.PERTURB)
+        while _uniform(bad) and ls > floor:
+            ls = max(floor, ls // 2)
+            UM, sM, VMH, bad = sketch(ls, linalg.PERTURB)
+        if _uniform(bad):
+            UM, sM, VMH, bad = sketch(ls, 1e-10)
+            if any(bad):
+                raise TrgError("leg-swap range finder: Cholesky breakdown on a rank-deficient sketch")
+    k = _uniform(max(min(c, chi_k), 1) for c in _count_above_b(sM, eps))
+    UM = ops.contiguous(UM[:, :, :k]).view(Bn, ni, ng, k)
+    VMH = ops.contiguous(VMH[:, :k]).view(Bn, k, nj, nb)
+    return UM, ops.contiguous(sM[:, :k]), VMH
+
+
+def _leg_swapping_b(B, C, Dcut):
+    """leg_swapping for a batch: B (B, i, t1, x1, y1), C (B, T0, X0, Y0, j)."""
+    Bn, ni, nt1, nx1, ny1 = B.shape
+    _, nT0, nX0, nY0, nj = C.shape
+    beta_ref = min(nx1 * ny1, ni * nt1)
+    gamma_ref = min(nT0 * nj, nX0 * nY0)
+    chi_k = min(Dcut, nj * beta_ref, ni * gamma_ref)
+    Bt = ops.permute(B, (0, 3, 4, 1, 2)).view(Bn, nx1 * ny1, ni, nt1)
+    Ct = ops.permute(C, (0, 1, 4, 2, 3)).view(Bn, nT0, nj, nX0 * nY0)
+    q = Dcut if RSVD["q"] is None else RSVD["q"]
+    UM, sM, VMH = _rsvd_leg_swap_b(Bt, Ct, RSVD["k_factor"] * chi_k, RSVD["p"], q, truncate_eps, RSVD["seed"],
+                                   keep=chi_k)
+    chi = _uniform(_truncate_b(sM, chi_k, degeneracy_eps, truncate_eps))
+    UM = ops.contiguous(UM[:, :, :, :chi]).view(Bn, ni, nX0, nY0, chi)
+    VMH = ops.contiguous(VMH[:, :chi]).view(Bn, chi, nj, nx1, ny1)
+    return UM, ops.contiguous(sM[:, :chi]), VMH
+
+
+def _squeezer_b(A, UM, sM, VMH, D, Dcut):
+    """squeezer for a batch: the environments in batched contractions, then
+    the oblique pairs of every instance through _oblique_pairs (its Cholesky
+    roots and SVDs run as launches over all instances)."""
+    dt = A.dtype
+    A, UM, VMH, D = _f64(A), _f64(UM), _f64(VMH), _f64(D)
+    Bn = A.shape[0]
+    nX1, nY1 = A.shape[2], A.shape[3]
+    nX0, nY0 = UM.shape[2], UM.shape[3]
+    nx1, ny1 = VMH.shape[3], VMH.shape[4]
+    nx0, ny0 = D.shape[3], D.shape[4]
+    k = sM.shape[1]
+    sMd = ops.cast(sM, A.dtype)
+    UMs = ops.contiguous(UM)
+    ops.scale_rows_(UMs, sMd, k, 1, 0, batch=Bn)
+    VMHs = ops.contiguous(VMH)
+    ops.scale_rows_(VMHs, sMd, k, VMH[0].numel() // k, 0, batch=Bn)
+    P = contract("Zaibc,Zakbf->Zickf", A, A)
+    Q = contract("Zcjde,Zflde->Zcjfl", UM, UMs)
+    XdX = contract("Zickf,Zcjfl->Zijkl", P, Q).reshape(Bn, nX1 * nX0, nX1 * nX0)
+    P = contract("Zabic,Zabkf->Zickf", A, A)
+    Q = contract("Zcdje,Zfdle->Zcjfl", UM, UMs)
+    YdY = contract("Zickf,Zcjfl->Zijkl", P, Q).reshape(Bn, nY1 * nY0, nY1 * nY0)
+    P = contract("Zbejf,Zdelf->Zbjdl", D, D)
+    Q = contract("Zabic,Zadkc->Zbidk", VMHs, VMH)
+    xxd = contract("Zbidk,Zbjdl->Zijkl", Q, P).reshape(Bn, nx1 * nx0, nx1 * nx0)
+    P = contract("Zbefj,Zdefl->Zbjdl", D, D)
+    Q = contract("Zabci,Zadck->Zbidk", VMHs, VMH)
+    yyd = contract("Zbidk,Zbjdl->Zijkl", Q, P).reshape(Bn, ny1 * ny0, ny1 * ny0)
+    items = []
+    for b in range(Bn):
+        items.append((XdX[b], xxd[b], (nX1, nX0), (nx1, nx0)))
+        items.append((YdY[b], yyd[b], (nY1, nY0), (ny1, ny0)))
+    pairs = _oblique_pairs(items, Dcut, dt)
+    _uniform((pairs[2 * b][0].shape[0], pairs[2 * b + 1][0].shape[0]) for b in range(Bn))
+    Px = _stack([pairs[2 * b][0] for b in range(Bn)])
+    PX = _stack([pairs[2 * b][1] for b in range(Bn)])
+    Py = _stack([pairs[2 * b + 1][0] for b in range(Bn)])
+    PY = _stack([pairs[2 * b + 1][1] for b in range(Bn)])
+    return Px, PX, Py, PY
+
+
+def _coarse_graining_b(A, UM, sM, VMH, D, Px, PX, Py, PY):
+    """coarse_graining for a batch."""
+    Bn = A.shape[0]
+    nT, nt = A.shape[1], D.shape[2]
+    nX, nY = PX.shape[3], PY.shape[3]
+    nx, ny = Px.shape[1], Py.shape[1]
+    nk = UM.shape[4]
+    W = contract("ZTabi,ZacX->ZTbicX", A, PX, tag="atrg_cg")
+    W = contract("ZTbicX,Zicdk->ZTbXdk", W, UM, tag="atrg_cg")
+    G = contract("ZTbXdk,ZbdY->ZTXYk", W, PY, tag="atrg_cg")
+    W = contract("Zkjab,Zxac->Zkjbxc", VMH, Px, tag="atrg_cg")
+    W = contract("Zkjbxc,Zjtcd->Zkbxtd", W, D, tag="atrg_cg")
+    H = contract("Zkbxtd,Zybd->Zktxy", W, Py, tag="atrg_cg")
+    del W
+    Gm, Hm = G.reshape(Bn, nT * nX * nY, nk), H.reshape(Bn, nk, nt * nx * ny)
+    rG, rH = min(nT * nX * nY, nk), min(nk, nt * nx * ny)
+    if BATCH_DECOMPOSITIONS and Gm.shape[1:] == Hm.shape[1:][::-1] and Gm.shape[1] >= Gm.shape[2]:
+        res = _by_shape(linalg.svd_thin, [_f64(Gm[b]) for b in range(Bn)] +
+                        [_f64(Hm[b]).transpose(0, 1) for b in range(Bn)])
+        uG, sGa, vhG = (_stack([r[i] for r in res[:Bn]]) for i in range(3))
+        uHt, sHa, vhHt = (_stack([r[i] for r in res[Bn:]]) for i in range(3))
+        uH, vhH = vhHt.transpose(1, 2), uHt.transpose(1, 2)
+    else:
+        uG, sGa, vhG = linalg.svd_thin(_f64(Gm))
+        uH, sHa, vhH = linalg.svd_thin(_f64(Hm))
+    cG = _uniform(_truncate_b(sGa, rG, degeneracy_eps, truncate_eps))
+    cH = _uniform(_truncate_b(sHa, rH, degeneracy_eps, truncate_eps))
+    UG, sG, VGH = ops.contiguous(uG[:, :, :cG]), ops.contiguous(sGa[:, :cG]), ops.contiguous(vhG[:, :cG])
+    UH, sH, VHH = ops.contiguous(uH[:, :, :cH]), ops.contiguous(sHa[:, :cH]), ops.contiguous(vhH[:, :cH])
+    # K = sG VGH sM UH sH
+    Kl = ops.contiguous(VGH)
+    ops.scale_rows_(Kl, sG, cG, Kl.shape[2], 0, batch=Bn)
+    ops.scale_rows_(Kl, ops.contiguous(sM), sM.shape[1], 1, 0, batch=Bn)
+    Kr = ops.contiguous(UH)
+    ops.scale_rows_(Kr, sH, cH, 1, 0, batch=Bn)
+    K = ops.gemm(Kl, Kr)
+    uK, sKa, vhK = linalg.svd_thin(_f64(K))
+    cK = _uniform(_truncate_b(sKa, min(K.shape[1:]), degeneracy_eps, truncate_eps))
+    UK, sK, VKH = ops.contiguous(uK[:, :, :cK]), ops.contiguous(sKa[:, :cK]), ops.contiguous(vhK[:, :cK])
+    dt = A.dtype
+    U = ops.cast(ops.gemm(UG, UK), dt).view(Bn, nT, nX, nY, -1)
+    VH = ops.cast(ops.gemm(VKH, VHH), dt).view(Bn, -1, nt, nx, ny)
+    return U, sK, VH
+
+
+def _pure_step_b(U, s, VH, Dcut, direction, prec):
+    dt = _dt(prec)
+    Bn = U.shape[0]
+    U, VH = ops.cast(U, dt), ops.cast(VH, dt)
+    U, VH = _leg_transposition_b(U, VH, "transpose", direction)
+    sd = ops.cast(s, dt)
+    ni = sd.shape[1]
+    Bm = ops.contiguous(VH)
+    ops.scale_rows_(Bm, sd, ni, VH[0].numel() // ni, 0, batch=Bn)      # s_i VH_{itxy}
+    C = ops.contiguous(U)
+    ops.scale_rows_(C, sd, ni, 1, 0, batch=Bn)                          # U_{TXYj} s_j
+    UM, sM, VMH = _leg_swapping_b(Bm, C, Dcut)
+    UM, VMH = ops.cast(UM, dt), ops.cast(VMH, dt)
+    Px, PX, Py, PY = _squeezer_b(U, UM, sM, VMH, VH, Dcut)
+    U2, s2, VH2 = _coarse_graining_b(U, UM, sM, VMH, VH, Px, PX, Py, PY)
+    return _leg_transposition_b(U2, VH2, "restore", direction) + (s2,)
+
+
+def atrg_pure_tensor_batch(U, s, VH, Dcut, direction, prec="f64"):
+    """:441-482 for a batch of instances of equal bond shapes: U (B,T,X,Y,i),
+    s (B,i), VH (B,i,t,x,y).  Returns [(U, s, VH)] per instance.  Every
+    kernel is batch-invariant, so each instance's result is the one it gets
+    alone; where the instances' ranks part (a truncation, the leg-swap eps
+    cut or a range-finder breakdown), the step is re-run for each group of
+    equal decisions."""
+    Bn = U.shape[0]
+    try:
+        U2, VH2, s2 = _pure_step_b(U, s, VH, Dcut, direction, prec)
+        return [(U2[b], s2[b], VH2[b]) for b in range(Bn)]
+    except RankSplit as e:
+        global SPLITS
+        SPLITS += 1
+        groups = {}
+        for b, key in enumerate(e.keys):
+            groups.setdefault(key, []).append(b)
+        out = [None] * Bn
+        for idx in groups.values():
+            sub = atrg_pure_tensor_batch(_stack([U[b] for b in idx]), _stack([s[b] for b in idx]),
+                                         _stack([VH[b] for b in idx]), Dcut, direction, prec)
+            for j, b in enumerate(idx):
+                out[b] = sub[j]
+        return out
+
+
+def normalize_batch(U, s, VH):
+    """normalize() for a batch (B, ...): returns (s, c) with c (B,) float64."""
+    Bn = U.shape[0]
+    d = contract("Ztxyi,Zitxy->Zi", _f64(U), _f64(VH))
+    c = ops.gemm(d.view(Bn, 1, -1), ops.contiguous(_f64(s)).view(Bn, -1, 1)).view(Bn)
+    s = ops.contiguous(_f64(s))
+    ops.normalize_(s, c, batch=Bn)
+    return s, c
 
 
 def trace(U, s, VH):

@@ -8,12 +8,15 @@ number of RG steps.  The server
     footprint alone exceeds the budget are rejected);
   * keeps every admitted flow's tensors resident on the GPU between steps;
   * runs RG steps on a small pool of HIP streams (one host thread each):
-    whenever a stream is free it takes the oldest ready flow together with
-    every other ready 2D HOTRG flow at the same step shape, direction and
-    precision, stacked into one batched launch sequence (ATRG flows, whose
-    bonds are dynamic, step individually); a flow is ready again as soon as
-    its step has finished, so the work of different requests overlaps on
-    the device and short flows are not held back by long ones;
+    whenever a stream is free it takes the ready flow of the earliest
+    request (so requests complete in arrival order rather than all
+    advancing together and finishing late) together with
+    every other ready flow of the same scheme at the same step shape,
+    direction and precision, stacked into one batched launch sequence (an
+    ATRG group whose rank decisions part re-runs the step per group of equal
+    decisions); a flow is ready again as soon as its step has finished, so
+    the work of different requests overlaps on the device and short flows
+    are not held back by long ones;
   * admits new arrivals whenever they come, as memory frees up;
   * records the latency arrival -> result of every request.
 
@@ -168,14 +171,21 @@ class Server:
                     f.step = n
                 keep += [T, c]
             else:
-                for f in flows:
-                    n = f.step + 1
-                    U, s, VH = atrg.atrg_pure_tensor(f.U, f.s, f.VH, f.req.chi, self._atrg_direction(n), "f64")
+                n0 = flows[0].step + 1
+                d = self._atrg_direction(n0)
+                if len(flows) > 1:
+                    # equal bond shapes: one batched step (re-run per group where ranks part)
+                    res = atrg.atrg_pure_tensor_batch(atrg._stack([f.U for f in flows]),
+                                                      atrg._stack([f.s for f in flows]),
+                                                      atrg._stack([f.VH for f in flows]), flows[0].req.chi, d, "f64")
+                else:
+                    res = [atrg.atrg_pure_tensor(flows[0].U, flows[0].s, flows[0].VH, flows[0].req.chi, d, "f64")]
+                for f, (U, s, VH) in zip(flows, res):
                     s, c = atrg.normalize(U, s, VH)
                     keep.append((f.U, f.s, f.VH))
                     f.U, f.s, f.VH = U, s, VH
                     f.consts.append(c)
-                    f.step = n
+                    f.step = n0
 
     def _finish(self, fl, now):
         r = fl.req
@@ -209,11 +219,12 @@ class Server:
         if f.scheme == "hotrg2d":
             n = f.step + 1
             return ("h", f.req.chi, tuple(f.T.shape[1:]), f.T.dtype, self._hotrg_direction(n), self.policy(n))
-        return ("a", f.req.rid)
+        n = f.step + 1
+        return ("a", f.req.chi, tuple(f.U.shape), tuple(f.s.shape), tuple(f.VH.shape), self._atrg_direction(n))
 
     def _take_group(self, ready):
-        """The oldest ready flow, with every other ready 2D flow that can share
-        its launches (same step shape, direction and precision)."""
+        """The oldest ready flow, with every other ready flow that can share
+        its launches (same scheme, step shape, direction and precision)."""
         key = self._group_key(ready[0])
         group, rest = [], []
         for f in ready:
@@ -232,6 +243,7 @@ class Server:
         now = lambda: time.perf_counter() - t0
         ready, running, done, used, qi = [], {}, [], 0, 0
         free = list(range(len(self.streams)))
+        last_note = 0.0
         while qi < len(queue) or ready or running:
             admitted = False
             while qi < len(queue) and (not realtime or queue[qi].arrival <= now()):
@@ -279,6 +291,15 @@ class Server:
                                 f"{res} latency={f.finished_at - f.req.arrival:.3f}s")
                     else:
                         ready.append(f)
+            # oldest request first: a stream always takes the earliest admitted ready flow (with
+            # every flow that can share its launches), so requests complete in arrival order
+            # instead of all advancing together
+            ready.sort(key=lambda f: (f.req.arrival, f.req.rid))
+            if log and now() - last_note >= 10.0:
+                last_note = now()
+                log(f"t={last_note:.1f}s: {len(done)} of {len(queue)} done, "
+                    f"{sum(len(g) for g, _ in running.values())} flows stepping, {len(ready)} ready, "
+                    f"{used / 2 ** 30:.1f} GiB admitted")
         return done
 
 
