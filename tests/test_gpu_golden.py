@@ -1,6 +1,7 @@
 """Parity at the benchmark configurations against golden vectors written by
 the float64 oracle (tests/golden/make_*.py): the device float64 path to 1e-12
-relative in ln Z/V and the benchmark's mixed-precision schedule to 1e-8.
+relative in ln Z/V and the benchmark's mixed-precision schedule to 1e-8
+(2D Ising chi = 64, 3D Ising chi = 16, 3D Potts ATRG).
 
 2D Ising HOTRG chi = 64, 40 RG steps (trg/HOTRG_2d.py:187-216): the oracle
 uses LAPACK eigh of the full 4096 x 4096 environments (:33-50); the device
@@ -70,6 +71,62 @@ def test_hotrg2d_chi64_golden_against_onsager():
         tol = 1e-4 if abs(beta - 0.4406867935) < 2e-3 else 1e-6
         assert abs(v["lnZ"] - exact.onsager_lnZ(beta)) < tol, (beta, v["lnZ"], exact.onsager_lnZ(beta))
         assert abs(sum(v["lnf"]) + np.log(v["trace"]) / 2 ** 40 - v["lnZ"]) < 1e-15
+
+
+# ---------------------------------------------------------------------------
+# Batched 3D Ising HOTRG chi = 16, 24 steps (trg/HOTRG_3d.py:247-322): the
+# oracle uses LAPACK eigh of the four 256 x 256 environments and a merge
+# sliced over the output leg; couplings of the benchmark's draw
+# ---------------------------------------------------------------------------
+GOLD3D = os.path.join(HERE, "golden", "hotrg3d_chi16.json")
+
+
+def _hotrg3d(gpu, policy):
+    import torch
+
+    from trg_backup_amd import hotrg, models
+    g = _load(GOLD3D)
+    betas = sorted(g["betas"], key=float)
+    nsteps = g["XLOOPS"] + g["YLOOPS"] + g["ZLOOPS"]
+    T0 = torch.stack([models.initial_tensor("ising3d", float(b), gpu) for b in betas])
+    print(f"hotrg3d chi={g['chi']} {nsteps} steps, betas {betas}", flush=True)
+    T, lnf = hotrg.renorm3d(T0, g["chi"], g["XLOOPS"], g["YLOOPS"], g["ZLOOPS"], policy)
+    z = hotrg.lnz3d(T, lnf, nsteps).cpu().numpy()
+    ref = np.array([g["betas"][b]["lnZ"] for b in betas])
+    lnf = lnf.cpu().numpy()
+    lref = np.array([g["betas"][b]["lnf"] for b in betas])
+    w = 2.0 ** np.arange(lref.shape[1])
+    print("max |d ln c_n| per step:", " ".join(f"{v:.0e}" for v in np.abs(lnf * w - lref * w).max(axis=0)), flush=True)
+    for b, a, r in zip(betas, z, ref):
+        print(f"beta={b}: device {a!r} oracle {r!r} diff {a - r:.3e} rel {abs(a - r) / abs(r):.3e}", flush=True)
+    return z, ref
+
+
+@pytest.mark.gpu
+def test_hotrg3d_chi16_fp64_matches_oracle(gpu):
+    from trg_backup_amd import hotrg
+    z, ref = _hotrg3d(gpu, hotrg.ALL_FP64)
+    assert np.all(np.abs(z - ref) <= 1e-12 * np.abs(ref))
+
+
+@pytest.mark.gpu
+def test_hotrg3d_chi16_bench_schedule_within_1e8_of_oracle(gpu):
+    from trg_backup_amd import workloads
+    z, ref = _hotrg3d(gpu, workloads.hotrg3d_policy())
+    assert np.all(np.abs(z - ref) <= 1e-8)
+
+
+def test_hotrg3d_golden_steps_consistent():
+    """The stored per-step constants reassemble the stored ln Z/V, the
+    couplings are the benchmark's draw, and the trace is positive."""
+    from trg_backup_amd import workloads
+    g = _load(GOLD3D)
+    draw = workloads.hotrg3d_betas(128)
+    for b, v in g["betas"].items():
+        assert float(b) == draw[v["draw_index"]]
+        assert v["trace"] > 0
+        assert abs(sum(v["lnf"]) + np.log(v["trace"]) / 2 ** 24 - v["lnZ"]) < 1e-15
+        assert 0.69 < v["lnZ"] < 0.85          # between ln 2 (beta = 0) and the ordered phase
 
 
 # ---------------------------------------------------------------------------
