@@ -242,8 +242,12 @@ def merge2d(T1, T2, Ua, Ub, compute=None):
         # epilogue stores four consecutive rows as one vector
         UamT, AT = Uam.transpose(1, 2), A.transpose(1, 2)
         if kinner:
-            # [c,k,i,e,b]: rows (i | b), columns (c,k | e)
-            gemm(UamT, AT, Z1p, compute=compute,
+            # [c,k,i,e,b]: rows (i | b), columns (c,k | e).  Z1's contraction is short (K = a), so
+            # its GEMM is bound by writing the chi^5 product, and its operands come M- and
+            # N-contiguous: the split tiers run it on the fp32 MFMA, whose kernel streams that
+            # layout as 16-byte vectors (and is at least as accurate as the split products)
+            z1c = "f32" if compute in ("bf16x3", "bf16x6") else compute
+            gemm(UamT, AT, Z1p, compute=z1c,
                  out_layout=([(chi_a, Dy * Dx), (Dx, 1)], [(nc * Dy, chi_a * Dy * Dx), (Dy, Dx)]))
             Z2a = Z1p.view(B, nc * Dy * chi_a, Dy * Dx)
         else:
