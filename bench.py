@@ -291,6 +291,11 @@ def bench_hotrg3d(args):
             roof["mfma_issue_frac"] = roof["frac"] * MFMA_PER_PRODUCT[last]
     else:
         roof = _roof(rec, "merge chi^11 GEMM (T.U1.U2 x T, f32 MFMA)", PEAK_TFLOPS["f32"])
+    tp = os.path.join(ROOT, "profiles", "gemm_merge3d_x3_traffic.json")
+    if roof and os.path.exists(tp):
+        # HBM bytes per launch (16 couplings) from the rocprof FETCH_SIZE / WRITE_SIZE runs of the probe
+        with open(tp) as f:
+            roof["traffic"] = json.load(f).get("bytes_per_launch", {}).get(last)
     print(json.dumps(_line(cfg, nb * K / dt, K, args.warmup, dt, dtype_label(policy, K), roof, extra)), flush=True)
     return 0
 
@@ -583,12 +588,53 @@ def gemm_sweep(args):
     return 0 if all(r["bitwise_equal_to_config_4"] for r in rows) else 1
 
 
+def _copy_calibration(dev):
+    """A plain 1 GiB float32 copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    import torch
+
+    from trg_backup_amd import ops
+    src = ops.fill_(torch.empty(1 << 28, dtype=torch.float32, device=dev), 1.0)
+    dst = torch.empty_like(src)
+    ops.copy_into(dst, src)
+    torch.cuda.synchronize()
+    print("[probe] copy of 1 GiB float32 done", flush=True)
+
+
+def traffic_probe_3d(args):
+    """The batched 3D solve of the bench on 16 couplings (one merge sub-batch) up to the first
+    steady three-piece split step (two chi^11 GEMM launches tuned, then counted), then the copy."""
+    import torch
+
+    from trg_backup_amd import _lib, autotune, hotrg, models, ops, workloads
+    _lib.load()
+    dev = torch.device("cuda:0")
+    betas = workloads.hotrg3d_betas(16)
+    policy = workloads.hotrg3d_policy(args.fp64_steps, args.merge3d)
+    T = ops.contiguous(torch.stack([models.initial_tensor("ising3d", b, dev) for b in betas]))
+    lnf = ops.fill_(torch.empty((len(betas), 16), dtype=torch.float64, device=dev), 0.0)
+    hotrg.normalize_trace(T, lnf, 0)
+    autotune.enable(not args.no_tune)
+    for n, d in workloads.schedule3d(args.fp64_steps + 1):
+        if n == args.fp64_steps + 1:
+            hotrg.step3d(T, args.chi, d, policy(n))      # tune first, then the step whose dispatches are read
+            torch.cuda.synchronize()
+            print(f"[probe] counted step {n} {d} merge={policy(n).merge}", flush=True)
+        T = hotrg.step3d(T, args.chi, d, policy(n))
+        hotrg.normalize_trace(T, lnf, n)
+        torch.cuda.synchronize()
+    autotune.enable(False)
+    _copy_calibration(dev)
+    return 0
+
+
 def traffic_probe(args):
-    """One steady step of the bench batch with each merge precision (float32, bf16x3, bf16),
-    then a plain 1 GiB float32 copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    """One steady step of the bench batch with each merge precision (float32, bf16x6, bf16x3,
+    bf16), then a plain 1 GiB float32 copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
     import torch
 
     from trg_backup_amd import _lib, autotune, hotrg, models, ops
+    if args.workload == "hotrg3d":
+        return traffic_probe_3d(args)
     _lib.load()
     dev = torch.device("cuda:0")
     chi, nb = args.chi, args.nbeta
@@ -614,11 +660,7 @@ def traffic_probe(args):
         T = hotrg.step2d(T, chi, d, prec)
         torch.cuda.synchronize()
     autotune.enable(False)
-    src = ops.fill_(torch.empty(1 << 28, dtype=torch.float32, device=dev), 1.0)
-    dst = torch.empty_like(src)
-    ops.copy_into(dst, src)
-    torch.cuda.synchronize()
-    print("[probe] copy of 1 GiB float32 done", flush=True)
+    _copy_calibration(dev)
     return 0
 
 
