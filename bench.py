@@ -60,7 +60,8 @@ def parse():
     ap.add_argument("--solve-steps", type=int, default=40, help="hotrg2d: RG steps of one solve (20 per axis)")
     ap.add_argument("--warmup", type=int, default=2, help="warm-up steps per precision tier")
     ap.add_argument("--chi", type=int, default=None, help="hotrg2d 64, hotrg3d 16, atrg3d 64")
-    ap.add_argument("--nbeta", type=int, default=None, help="hotrg2d 32, hotrg3d 32 (of the 128-value draw)")
+    ap.add_argument("--nbeta", type=int, default=None,
+                    help="hotrg2d 32, hotrg3d 128 (of the 128-value draw), atrg3d 1 (more: a batched sweep around 0.55)")
     ap.add_argument("--fp64-steps", type=int, default=None,
                     help="hotrg2d 6, hotrg3d 4, atrg3d 14, hotrg3d_qr all (its oblique projectors amplify "
                          "float32 rounding: f32 after step 4 gives 5e-8)")
@@ -383,21 +384,46 @@ def bench_atrg3d(args):
     saved = (dict(atrg.RSVD), atrg.BATCH_DECOMPOSITIONS, atrg.degeneracy_eps)
     atrg.degeneracy_eps = args.atrg_deg_eps
 
-    def solve(nsteps, prec, q, fp64_steps, what):
+    nb = args.nbeta
+    # one coupling (the configuration's 0.55), or nb couplings around it stepped as one batch
+    betas = [0.55] if nb == 1 else [round(0.55 + 0.005 * (i - (nb - 1) // 2), 6) for i in range(nb)]
+
+    def solve(nsteps, prec, q, fp64_steps, what, betas=(0.55,)):
+        """ln Z/V of each coupling; several couplings step together through
+        atrg_pure_tensor_batch, grouped by bond shape."""
         atrg.set_rsvd_params(k_factor=1, p=chi, q=q, seed=1)
-        U, s, VH = models.atrg_initial("potts3d", 0.55)
-        s, c = atrg.normalize(U, s, VH)
-        consts = [c]
+        st = []
+        for b in betas:
+            U, s, VH = models.atrg_initial("potts3d", b)
+            s, c = atrg.normalize(U, s, VH)
+            st.append([U, s, VH, [c]])
         t0 = time.perf_counter()
         for n in range(1, nsteps + 1):
-            U, s, VH = atrg.atrg_pure_tensor(U, s, VH, chi, "TXY"[(n - 1) % 3], "f64" if n <= fp64_steps else prec)
-            s, c = atrg.normalize(U, s, VH)
-            consts.append(c)
+            d, pr = "TXY"[(n - 1) % 3], "f64" if n <= fp64_steps else prec
+            if len(st) == 1:
+                res = [atrg.atrg_pure_tensor(st[0][0], st[0][1], st[0][2], chi, d, pr)]
+            else:
+                groups, res = {}, [None] * len(st)
+                for i, x in enumerate(st):
+                    groups.setdefault((tuple(x[0].shape), tuple(x[1].shape), tuple(x[2].shape)), []).append(i)
+                for idx in groups.values():
+                    out = atrg.atrg_pure_tensor_batch(atrg._stack([st[i][0] for i in idx]),
+                                                      atrg._stack([st[i][1] for i in idx]),
+                                                      atrg._stack([st[i][2] for i in idx]), chi, d, pr)
+                    for j, i in enumerate(idx):
+                        res[i] = out[j]
+            for x, (U, s, VH) in zip(st, res):
+                s, c = atrg.normalize(U, s, VH)
+                x[0], x[1], x[2] = U, s, VH
+                x[3].append(c)
             if n % 5 == 0 or n == nsteps:
                 log(f"{what}: step {n} at {time.perf_counter() - t0:.1f} s")
-        tr = atrg.trace(U, s, VH)
-        lg = sum(math.log(abs(float(c[0].item()))) / 2 ** k for k, c in enumerate(consts))
-        return lg + math.log(abs(float(tr[0].item()))) / 2 ** nsteps
+        zs = []
+        for U, s, VH, consts in st:
+            tr = atrg.trace(U, s, VH)
+            lg = sum(math.log(abs(float(c[0].item()))) / 2 ** k for k, c in enumerate(consts))
+            zs.append(lg + math.log(abs(float(tr[0].item()))) / 2 ** nsteps)
+        return zs
 
     try:
         autotune.enable(not args.no_tune)
@@ -405,8 +431,10 @@ def bench_atrg3d(args):
         # the bonds reach chi within the first few steps: the warm-up runs far enough into the
         # steady shapes that the tuner meets every GEMM problem of both precision tiers
         nw = min(K, max(8, 3 + args.warmup))
-        solve(nw, args.atrg_prec, args.atrg_q, nw, "warm-up")
-        solve(nw, args.atrg_prec, args.atrg_q, 0, "warm-up")
+        solve(nw, args.atrg_prec, args.atrg_q, nw, "warm-up", betas)
+        solve(nw, args.atrg_prec, args.atrg_q, 0, "warm-up", betas)
+        if not args.no_plain and nb > 1:
+            solve(nw, args.atrg_prec, args.atrg_q, nw, "warm-up (one coupling)")
         torch.cuda.synchronize()
         autotune.enable(False)
         log(f"warm-up in {time.perf_counter() - t0:.1f} s")
@@ -414,20 +442,25 @@ def bench_atrg3d(args):
         rec = ops.GemmRecorder("atrg_cg")
         t0 = time.perf_counter()
         with rec:
-            z = solve(K, args.atrg_prec, args.atrg_q, args.fp64_steps, "timed")
+            zs = solve(K, args.atrg_prec, args.atrg_q, args.fp64_steps, "timed", betas)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        log(f"timed {K} steps in {dt:.2f} s, ln Z/V = {z!r}")
+        z = zs[betas.index(0.55)]
+        log(f"timed {K} steps x {nb} couplings in {dt:.2f} s, ln Z/V(0.55) = {z!r}")
         extra = {"lnZ_per_site": {"beta": 0.55, "value": z}}
+        if nb > 1:
+            extra["lnZ_per_site"] = {"betas": betas, "values": zs}
+            extra["atrg_rank_splits"] = atrg.SPLITS
         if not args.no_plain:
             atrg.BATCH_DECOMPOSITIONS = False
             t1 = time.perf_counter()
-            zd = solve(K, "f64", args.atrg_q, K, "plain float64")
+            zd = solve(K, "f64", args.atrg_q, K, "plain float64")[0]
             tp = time.perf_counter() - t1
             atrg.BATCH_DECOMPOSITIONS = True
-            extra["plain_fp64"] = {"steps_per_s": K / tp, "solve_s": tp, "value": zd,
-                                   "path": "float64 contractions, decompositions one at a time, host syncs per step"}
-            extra["speedup_vs_plain_fp64"] = (K / dt) / (K / tp)
+            extra["plain_fp64"] = {"steps_per_s": K / tp, "solve_s": tp, "value": zd, "beta": 0.55,
+                                   "path": "one coupling, float64 contractions, decompositions one at a time, "
+                                           "host syncs per step"}
+            extra["speedup_vs_plain_fp64"] = (nb * K / dt) / (K / tp)
             extra["lnZ_err_vs_fp64"] = abs(z - zd)
     finally:
         atrg.RSVD.update(saved[0])
@@ -455,13 +488,16 @@ def bench_atrg3d(args):
                 "by_precision": byp,
                 "note": "the rest of the step is float64 decompositions (CholeskyQR range finders, Jacobi "
                         "SVDs, PSD Cholesky roots), latency-bound; see the rocprof summary in profiles/"}
-    cfg = {"workload": f"3D q=3 Potts ATRG chi={chi}, beta=0.55, {K} RG steps (T,X,Y), rSVD k=chi p=chi "
-                       f"q={args.atrg_q} seed 1", "chi": chi, "rg_steps": K,
+    sweep = "beta=0.55" if nb == 1 else f"{nb} couplings beta in [{betas[0]}, {betas[-1]}] batched"
+    cfg = {"workload": f"3D q=3 Potts ATRG chi={chi}, {sweep}, {K} RG steps (T,X,Y), rSVD k=chi p=chi "
+                       f"q={args.atrg_q} seed 1", "chi": chi, "rg_steps": K, "nbeta": nb,
            "DGENERACY_EPS": args.atrg_deg_eps,
            "precision": f"f64 for steps <= {args.fp64_steps}, then {args.atrg_prec} contractions "
                         "(decompositions and squeezer environments f64)",
-           "parallelism": "single GPU, one instance, independent decompositions batched"}
-    print(json.dumps(_line(cfg, K / dt, K, args.warmup, dt, args.atrg_prec, roof, extra)), flush=True)
+           "parallelism": ("single GPU, one instance, independent decompositions batched" if nb == 1 else
+                           "single GPU, couplings of equal bond shapes stepped as one batch")}
+    dtype = f"mixed (steps 1-{args.fp64_steps}: f64; after: {args.atrg_prec} contractions; f64 decompositions)"
+    print(json.dumps(_line(cfg, nb * K / dt, K, args.warmup, dt, dtype, roof, extra)), flush=True)
     return 0
 
 
