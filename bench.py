@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--nbeta", type=int, default=None,
                     help="hotrg2d 32, hotrg3d 128 (of the 128-value draw), atrg3d 1 (more: a batched sweep around 0.55)")
     ap.add_argument("--fp64-steps", type=int, default=None,
-                    help="hotrg2d 6, hotrg3d 4, atrg3d 14, hotrg3d_qr all (its oblique projectors amplify "
+                    help="hotrg2d 6, hotrg3d 6, atrg3d 14, hotrg3d_qr all (its oblique projectors amplify "
                          "float32 rounding: f32 after step 4 gives 5e-8)")
     ap.add_argument("--atrg-q", type=int, default=2)
     ap.add_argument("--squeezer", default="QR", choices=["original", "QR", "fix"], help="hotrg3d_qr squeezer")
@@ -81,7 +81,9 @@ def parse():
     ap.add_argument("--split-merge-from", type=int, default=10, help="bf16x3 merge after this step (-1: never)")
     ap.add_argument("--bf16-env-from", type=int, default=12, help="bf16 environments after this step (-1: never)")
     ap.add_argument("--bf16-merge-from", type=int, default=22, help="bf16 merge after this step (-1: never)")
-    ap.add_argument("--h3-split-from", type=int, default=-1, help="hotrg3d: bf16x3 merges after this step (-1: never)")
+    ap.add_argument("--h3-split-from", type=int, default=16, help="hotrg3d: bf16x3 merges after this step (-1: never)")
+    ap.add_argument("--h3-validate", type=int, default=8,
+                    help="hotrg3d: couplings re-solved in float64 for the accuracy check (spread over the batch)")
     ap.add_argument("--h3-bf16-from", type=int, default=-1, help="hotrg3d: bf16 merges after this step (-1: never)")
     ap.add_argument("--h3-bf16-env-from", type=int, default=-1,
                     help="hotrg3d: bf16 environments after this step (-1: never)")
@@ -140,7 +142,8 @@ def bench_hotrg3d(args):
     """Batched 3D Ising HOTRG: beta drawn from numpy.random.default_rng(0).uniform(0.20, 0.24, 128)
     (the first --nbeta of them), chi = 16, steps Z, Y, X (trg/HOTRG_3d.py:259-316) from the initial
     tensor; float64 for the first --fp64-steps steps, then three-piece split-bf16 GEMMs (fp32 accuracy;
-    --merge3d f32 for the float32 MFMA) with float64 eigensolvers."""
+    --merge3d f32 for the float32 MFMA), split-bf16 merges after --h3-split-from, with float64
+    eigensolvers; --h3-validate couplings are re-solved in float64 for the accuracy check."""
     import numpy as np
     import torch
 
@@ -203,7 +206,9 @@ def bench_hotrg3d(args):
 
     def run_timed():
         if not graphed:
-            return run(sched, policy, T0)
+            # a batched chi = 16 step is seconds of GPU work: the per-step sync of --progress costs
+            # nothing measurable
+            return run(sched, policy, T0, "timed" if args.progress else None)
         ops.fill_(lnf_g, 0.0)
         T = ops.contiguous(T0, policy(0).store)
         hotrg.normalize_trace(T, lnf_g, 0)
@@ -243,7 +248,8 @@ def bench_hotrg3d(args):
     torch.cuda.empty_cache()
     extra = {"lnZ_per_site": {"betas": betas[:3], "values": zt[:3].tolist()}}
     if not args.no_validate:
-        sel = [0, 1, nb - 1] if nb > 2 else list(range(nb))
+        nv = max(1, min(nb, args.h3_validate))
+        sel = sorted({round(i * (nb - 1) / max(1, nv - 1)) for i in range(nv)} | {0})
         Td, ld = run(sched, hotrg.ALL_FP64, ops.contiguous(T0[sel]), "float64 validation")
         zd = hotrg.lnz3d(Td, ld, K).cpu().numpy()
         errs = [abs(float(zt[i]) - float(zd[j])) for j, i in enumerate(sel)]
@@ -734,7 +740,7 @@ def main():
     args.steps_requested = args.steps
     args.steps = args.steps or {"hotrg2d": 40, "hotrg3d": 24, "hotrg3d_qr": 6, "atrg3d": 30}.get(wl, 40)
     args.fp64_steps = (args.fp64_steps if args.fp64_steps is not None
-                       else {"hotrg2d": 6, "hotrg3d": 4, "hotrg3d_qr": 10 ** 6, "atrg3d": 14}.get(wl, 0))
+                       else {"hotrg2d": 6, "hotrg3d": 6, "hotrg3d_qr": 10 ** 6, "atrg3d": 14}.get(wl, 0))
     if args.gemm_sweep:
         return gemm_sweep(args)
     if args.traffic_probe:
