@@ -194,6 +194,7 @@ def squeezer_environments(A, UM, sM, VMH, D):
 
 
 EIG_Q = 12   # power iterations of the top-k SVD of RX Rx when it is too large for the dense path
+EIG_L = 2    # its block: EIG_L x the kept rank (at most linalg.SMALL_MAX columns)
 # Run the squeezer's independent decompositions (four Cholesky roots, two SVDs)
 # as batched launches.  Every kernel is batch-invariant, so each result is the
 # same bits either way; False reproduces the reference's one-at-a-time order.
@@ -247,7 +248,7 @@ def _oblique_pairs(items, Dcut, dt, deg_eps=None, trunc_eps=None):
     for i in range(len(Ps)):
         if not small[i]:
             n = Ps[i].shape[0]
-            topk.setdefault(min(n, max(2 * k1s[i], k1s[i] + 8), linalg.SMALL_MAX), []).append(i)
+            topk.setdefault(min(n, max(EIG_L * k1s[i], k1s[i] + 8), linalg.SMALL_MAX), []).append(i)
     for l, idx in topk.items():
         for i, r in zip(idx, _by_shape(lambda A: linalg.svd_topk(A, l, EIG_Q), [Ps[i] for i in idx])):
             svds[i] = r
