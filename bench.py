@@ -8,11 +8,12 @@ the schedule (K = 40 by default: bond growth, float64 steps, float32 steps, bf16
 with the inputs resident in HBM, then ln Z/V, closed by a device synchronisation.
 value = 32 * K / (timed seconds)  [RG steps/s, whole GPU].
 
-Precision schedule (hotrg.Policy): steps 1..6 in float64, then float32 storage with float64
-isometry eigensolvers, traces and log-norms; the merge GEMMs run on float32 MFMA up to step 10,
-on split-bf16 ("bf16x3", about 16 significant bits per operand) up to step 22 and on bf16 after
-that, the environment GEMMs on bf16 after step 12.  An error made at step n enters ln Z/V with
-weight 2^-n, which is what lets later steps run at lower precision.
+Precision schedule (hotrg.Policy, workloads.hotrg2d_policy): steps 1..6 in float64, then float32
+storage with float64 isometry eigensolvers, traces and log-norms; the merge GEMMs run on the
+three-piece split-bf16 ("bf16x6", fp32 accuracy on six bf16 MFMAs) up to step 10, on split-bf16
+("bf16x3", about 16 significant bits per operand) up to step 22 and on bf16 after that; the
+environment GEMMs on bf16x6 up to step 12 and on bf16 after.  An error made at step n enters
+ln Z/V with weight 2^-n, which is what lets later steps run at lower precision.
 
 Warm-up (untimed): the same schedule with every precision tier after step 6 cut to its first
 W steps, which loads every kernel and lets the GEMM autotuner time each tile configuration on
@@ -22,8 +23,7 @@ Also on the JSON line:
   * roofline: the kernel with the most time in the timed solve (the chi^7 merge GEMM), its
     launches timed live with HIP events on the launching stream; algorithmic flops 2 M N K per
     launch; the merge GEMMs of the other precisions are listed beside it;
-  * lnZ_err_vs_fp64: max |ln Z/V (timed solve) - ln Z/V (float64 solve)| over a subset of the
-    couplings (all of them with --validate-all);
+  * lnZ_err_vs_fp64: max |ln Z/V (timed solve) - ln Z/V (float64 solve)| over all 32 couplings;
   * plain_fp64: one instance, float64 throughout, a host sync per step, the same solve (the
     per-instance float64 GPU path); speedup_vs_plain_fp64 = value / its steps/s;
   * serving: a Poisson stream of mixed solves through serve.Server (p50/p99 latency);
