@@ -106,3 +106,35 @@ def test_merge3d_chain_layouts(monkeypatch, shape, compute, chunks):
     out = hotrg.merge3d(T1, T2, U1a, U1b, U2a, U2b, compute=compute)
     assert out.shape == ref.shape
     assert (out - ref).abs().max().item() < 1e-10 * ref.abs().max().item()
+
+
+def test_atrg_batch_regroups_where_decisions_part(monkeypatch):
+    """atrg.atrg_pure_tensor_batch re-runs a step per group of equal rank decisions (recursively, as
+    often as later decisions part again) and returns each instance's result in input order; the
+    batched step is emulated on the CPU with the instances' decision keys in s[:, 0] and a second
+    decision in s[:, 1] that parts only inside the first group."""
+    from trg_backup_amd import atrg
+    calls = []
+
+    def fake_step(U, s, VH, Dcut, direction, prec):
+        calls.append(s[:, :2].tolist())
+        for col in (0, 1):
+            keys = [int(v) for v in s[:, col].tolist()]
+            if len(set(keys)) > 1:
+                raise atrg.RankSplit(keys)
+        return U + 1.0, VH * 2.0, s * 3.0          # (U, VH, s), the order _pure_step_b returns
+
+    monkeypatch.setattr(atrg, "_pure_step_b", fake_step)
+    monkeypatch.setattr(atrg, "_stack", lambda ts: torch.stack(list(ts)))
+    s = torch.tensor([[1., 5.], [2., 5.], [1., 6.], [3., 5.], [1., 5.]])
+    U = torch.arange(10, dtype=torch.float64).view(5, 2)
+    VH = -torch.arange(15, dtype=torch.float64).view(5, 3)
+    out = atrg.atrg_pure_tensor_batch(U, s.double(), VH, 4, "T")
+    assert len(out) == 5
+    for b in range(5):
+        u, sv, vh = out[b]
+        assert torch.equal(u, U[b] + 1.0) and torch.equal(vh, VH[b] * 2.0) and torch.equal(sv, s[b].double() * 3.0)
+    # the whole batch, then the groups of the first decision {0,2,4}, {1}, {3}; the first of them
+    # parts again on the second decision into {0,4} and {2}
+    assert [len(c) for c in calls] == [5, 3, 2, 1, 1, 1]
+    assert calls[2] == [[1.0, 5.0], [1.0, 5.0]]
