@@ -81,6 +81,8 @@ def parse():
     ap.add_argument("--split-merge-from", type=int, default=10, help="bf16x3 merge after this step (-1: never)")
     ap.add_argument("--bf16-env-from", type=int, default=12, help="bf16 environments after this step (-1: never)")
     ap.add_argument("--bf16-merge-from", type=int, default=22, help="bf16 merge after this step (-1: never)")
+    ap.add_argument("--eig-q-tiers", default="12:8,22:4",
+                    help="hotrg2d: isometry eigensolver iterations after a step, as step:q pairs ('none': 12 throughout)")
     ap.add_argument("--h3-split-from", type=int, default=16, help="hotrg3d: bf16x3 merges after this step (-1: never)")
     ap.add_argument("--h3-validate", type=int, default=8,
                     help="hotrg3d: couplings re-solved in float64 for the accuracy check (spread over the batch)")
@@ -765,8 +767,10 @@ def main():
     nsolve = max(1, -(-args.steps // K))
     chi, nb = args.chi, args.nbeta
     betas = workloads.hotrg2d_betas(nb)
+    eig_q = () if args.eig_q_tiers == "none" else tuple(tuple(int(x) for x in t.split(":"))
+                                                          for t in args.eig_q_tiers.split(","))
     policy = workloads.hotrg2d_policy(args.fp64_steps, args.split6_env_from, args.split6_merge_from,
-                                      args.split_merge_from, args.bf16_env_from, args.bf16_merge_from)
+                                      args.split_merge_from, args.bf16_env_from, args.bf16_merge_from, eig_q)
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     T0 = torch.stack([models.initial_tensor("ising2d", b, dev) for b in betas])
 

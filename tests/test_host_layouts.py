@@ -63,6 +63,11 @@ def test_precision_policy_tiers():
     assert (pol(23).env, pol(23).merge) == ("bf16", "bf16")
     assert pol(7).env_compute() == "bf16x6" and pol(7).store == torch.float32
     assert "bf16x6 env after step 6" in pol.describe()
+    # eigensolver iteration tiers: 0 (the module setting) until the first tier
+    pol = hotrg.Policy(fp64_steps=6, low=hotrg.FP32, bf16_merge_from=22, eig_q=((12, 8), (22, 4)))
+    assert pol(6).eig_q == 0 and pol(12).eig_q == 0 and pol(13).eig_q == 8 and pol(23).eig_q == 4
+    assert pol(7) is hotrg.FP32 and pol(23).merge == "bf16"
+    assert "4 eigensolver iterations after step 22" in pol.describe()
 
 
 class _CpuOps:

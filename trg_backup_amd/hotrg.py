@@ -45,10 +45,12 @@ class Precision:
     the merge ("f64", "f32", "bf16x6" = three-piece split-bf16 products with
     the 24 bits of fp32, "bf16x3" = split-bf16 products with about 16
     significant bits per operand, "bf16").  Eigensolvers, traces and
-    log-norms are always float64."""
+    log-norms are always float64; eig_q > 0 sets the isometry eigensolver's
+    subspace iterations for the step (0: the module setting EIG / EIG3)."""
     store: torch.dtype = torch.float64
     env: str = "f64"
     merge: str = "f64"
+    eig_q: int = 0
 
     def env_compute(self):
         return None if self.store == torch.float64 else (self.env if self.env in SPLIT_COMPUTE else None)
@@ -68,8 +70,10 @@ class Policy:
     early steps run in float64 and later ones in lower precision."""
 
     def __init__(self, fp64_steps=10**9, low=FP32, bf16_env_from=None, bf16_merge_from=None,
-                 split_merge_from=None, split6_merge_from=None, split6_env_from=None):
+                 split_merge_from=None, split6_merge_from=None, split6_env_from=None, eig_q=()):
         self.fp64_steps = fp64_steps
+        # ((after_step, q), ...): fewer eigensolver iterations once a step's error weighs little
+        self.eig_q = tuple(eig_q or ())
         self.low = low
         self.split6_env_from = split6_env_from
         self.bf16_env_from = bf16_env_from
@@ -91,9 +95,13 @@ class Policy:
             merge = "bf16x3"
         if self.bf16_merge_from is not None and step > self.bf16_merge_from:
             merge = "bf16"
-        if (env, merge) == (self.low.env, self.low.merge):
+        q = 0
+        for after, qq in self.eig_q:
+            if step > after:
+                q = qq
+        if (env, merge, q) == (self.low.env, self.low.merge, self.low.eig_q):
             return self.low
-        return Precision(self.low.store, env, merge)
+        return Precision(self.low.store, env, merge, q)
 
     def describe(self):
         s = f"fp64 for steps <= {self.fp64_steps}, then {self.low.store} env={self.low.env} merge={self.low.merge}"
@@ -107,6 +115,8 @@ class Policy:
             s += f", bf16x3 merge after step {self.split_merge_from}"
         if self.bf16_merge_from is not None:
             s += f", bf16 merge after step {self.bf16_merge_from}"
+        for after, q in self.eig_q:
+            s += f", {q} eigensolver iterations after step {after}"
         return s
 
 
@@ -182,7 +192,7 @@ def projector2d(T, Dcut, prec=FP64):
     MM = torch.empty((2 * B, n, n), dtype=torch.float64, device=T.device)
     env2d(T, 0, c, out=MM[:B])
     env2d(T, 1, c, out=MM[B:])
-    U, flag = _pick(MM, B, Dcut, T.dtype)
+    U, flag = _pick(MM, B, Dcut, T.dtype, dict(EIG, q=prec.eig_q) if prec.eig_q else None)
     return U.view(B, Dx, Dx, Dcut), flag
 
 

@@ -20,16 +20,19 @@ def hotrg2d_betas(nb=32):
 
 
 def hotrg2d_policy(fp64_steps=6, split6_env_from=6, split6_merge_from=6, split_merge_from=10,
-                   bf16_env_from=12, bf16_merge_from=22):
+                   bf16_env_from=12, bf16_merge_from=22, eig_q=((12, 8), (22, 4))):
     """float64 through step 6, then float32 storage with: three-piece split-bf16
     (fp32 accuracy) environments for steps 7-12 and merges for 7-10, split-bf16
-    merges for 11-22, bf16 merges after 22 and bf16 environments after 12
-    (a value below 0 switches a tier off).  Worst |d ln Z/V| against float64
-    over the 32 couplings: 5.9e-9."""
+    merges for 11-22, bf16 merges after 22 and bf16 environments after 12 (a
+    value below 0 switches a tier off); the isometry eigensolver runs its 12
+    subspace iterations through step 12, 8 through step 22 and 4 after (an
+    isometry error enters ln Z/V at second order and with the step's weight
+    2^-n; eig_q=() keeps 12 everywhere)."""
     opt = lambda v: None if v is None or v < 0 else v
     return hotrg.Policy(fp64_steps=fp64_steps, low=hotrg.FP32, bf16_env_from=opt(bf16_env_from),
                         split_merge_from=opt(split_merge_from), bf16_merge_from=opt(bf16_merge_from),
-                        split6_merge_from=opt(split6_merge_from), split6_env_from=opt(split6_env_from))
+                        split6_merge_from=opt(split6_merge_from), split6_env_from=opt(split6_env_from),
+                        eig_q=eig_q)
 
 
 def hotrg3d_betas(nb=128):
