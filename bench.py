@@ -933,7 +933,12 @@ def main():
         t1 = time.perf_counter()
         Tc, _ = oh2.__tensor_normalization__(oh2.new_pure_tensor(Tc, chi, dd))
         tc = time.perf_counter() - t1
-        out["cpu_baseline"] = {"value": 1.0 / tc, "unit": "RG steps/s", "cores": os.cpu_count(), "kind": "port",
+        try:
+            from threadpoolctl import threadpool_info
+            cores = max(d["num_threads"] for d in threadpool_info() if d.get("user_api") == "blas")
+        except (ImportError, ValueError):
+            cores = os.cpu_count()
+        out["cpu_baseline"] = {"value": 1.0 / tc, "unit": "RG steps/s", "cores": cores, "kind": "port",
                                "sample": f"one steady-state chi={chi} {dd} step of one instance (beta=0.44), "
                                          f"NumPy/OpenBLAS float64 oracle, {tc:.1f} s"}
     print(json.dumps(out), flush=True)
