@@ -84,6 +84,8 @@ def parse():
     ap.add_argument("--eig-q-tiers", default="12:8,22:4",
                     help="hotrg2d: isometry eigensolver iterations after a step, as step:q pairs ('none': 12 throughout)")
     ap.add_argument("--h3-split-from", type=int, default=16, help="hotrg3d: bf16x3 merges after this step (-1: never)")
+    ap.add_argument("--h3-fp64-merge", choices=("f64", "f64i8"), default="f64i8",
+                    help="hotrg3d: merges of the float64 steps on the fp64 MFMA or as int8-MFMA slice products")
     ap.add_argument("--h3-validate", type=int, default=8,
                     help="hotrg3d: couplings re-solved in float64 for the accuracy check (spread over the batch)")
     ap.add_argument("--h3-bf16-from", type=int, default=-1, help="hotrg3d: bf16 merges after this step (-1: never)")
@@ -155,7 +157,7 @@ def bench_hotrg3d(args):
     chi, nb, K = args.chi, args.nbeta, args.steps
     betas = workloads.hotrg3d_betas(nb)
     policy = workloads.hotrg3d_policy(args.fp64_steps, args.merge3d, args.h3_split_from, args.h3_bf16_from,
-                                      args.h3_bf16_env_from)
+                                      args.h3_bf16_env_from, fp64_merge=args.h3_fp64_merge)
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     T0 = torch.stack([models.initial_tensor("ising3d", b, dev) for b in betas])
     sched = workloads.schedule3d(K)
@@ -171,7 +173,7 @@ def bench_hotrg3d(args):
             hotrg.normalize_trace(T, lnf, n)
             if what:
                 torch.cuda.current_stream().synchronize()
-                log(f"{what}: step {n} {d} done")
+                log(f"{what}: step {n} {d} done at {time.perf_counter() - t0:.1f} s")
         return T, lnf
 
     autotune.enable(not args.no_tune)
@@ -298,6 +300,13 @@ def bench_hotrg3d(args):
                           f"last tier of the solve)", PEAK_TFLOPS[last])
         if roof:
             roof["mfma_issue_frac"] = roof["frac"] * MFMA_PER_PRODUCT[last]
+    elif last == "f64i8":
+        # 21 int8 products per float64 product: the int8 peak over 21 is this GEMM's ceiling
+        roof = _roof(rec, "merge chi^11 GEMM (T.U1.U2 x T, float64 as 6-slice int8 MFMA products)", 5000.0 / 21)
+        if roof:
+            roof["mfma_issue_frac"] = roof["frac"]
+    elif last == "f64":
+        roof = _roof(rec, "merge chi^11 GEMM (T.U1.U2 x T, f64 MFMA)", 78.6)
     else:
         roof = _roof(rec, "merge chi^11 GEMM (T.U1.U2 x T, f32 MFMA)", PEAK_TFLOPS["f32"])
     tp = os.path.join(ROOT, "profiles", "gemm_merge3d_x3_traffic.json")
@@ -713,7 +722,10 @@ def dtype_label(policy, nsteps):
     tiers, last = [], None
     for n in range(1, nsteps + 1):
         p = policy(n)
-        key = "f64" if p.store.is_floating_point and p.store.itemsize == 8 else f"env {p.env}, merge {p.merge}"
+        if p.store.is_floating_point and p.store.itemsize == 8:
+            key = "f64 (merges: fp64 products on the int8 MFMA, 6 slices)" if p.merge == "f64i8" else "f64"
+        else:
+            key = f"env {p.env}, merge {p.merge}"
         if key != last:
             tiers.append([n, n, key])
             last = key

@@ -80,6 +80,30 @@ int trg_gemm_ex(int config, int dtype, int c_dtype, int compute, int batch, int 
                 double beta, void *C, long long sCb, long long rsC, long long csC,
                 int cdm, long long csm, int cdn, long long csn, void *stream);
 int trg_gemm_config_count(int dtype, int c_dtype, int compute);
+
+/*
+ * Float64 GEMM on the int8 matrix cores, same operands, strides and output
+ * layout as trg_gemm_ex with dtype F64 (sAb / sBb = 0: one operand shared by
+ * the batch).  Every row of A and column of B becomes 2^e times `slices`
+ * (4..7) signed 7-bit integer slices; the slice products are summed exactly
+ * in int32 on v_mfma_i32_32x32x32_i8 and the slices(slices+1)/2 products
+ * whose weight reaches 2^-7(slices+1) are combined in fp64.  The product is
+ * within about 2^-7 slices of (row max of |A|) x (column max of |B|) x K per
+ * entry (slices = 6: 21 int8 MFMAs per product, 2^-42).  A non-finite entry
+ * makes its row or column of C NaN.  The workspace holds the slices and the
+ * row exponents, trg_gemm_f64i8_workspace bytes; K beyond the int32 range of
+ * the sums (slices 127^2 K < 2^31) runs in chunks added in fp64.  The result
+ * depends on nothing but the inputs and the K chunking: no tile configuration
+ * or batch enters it.  Replaces the same contraction steps as trg_gemm in
+ * its float64 form: trg/HOTRG_3d.py:107-116, trg/HOTRG_2d.py:65-67.
+ */
+long long trg_gemm_f64i8_workspace(int slices, int batchA, int batchB, int M, int N, int K);
+int trg_gemm_f64i8(int slices, int batch, int M, int N, int K, double alpha,
+                   const void *A, long long sAb, long long rsA, long long csA,
+                   const void *B, long long sBb, long long rsB, long long csB,
+                   double beta, void *C, long long sCb, long long rsC, long long csC,
+                   int cdm, long long csm, int cdn, long long csn,
+                   void *workspace, long long ws_bytes, void *stream);
 const char *trg_gemm_config_name(int dtype, int c_dtype, int compute, int config);
 
 /*

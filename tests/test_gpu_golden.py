@@ -110,9 +110,12 @@ def test_hotrg3d_chi16_fp64_matches_oracle(gpu):
 
 
 @pytest.mark.gpu
-def test_hotrg3d_chi16_bench_schedule_within_1e8_of_oracle(gpu):
+@pytest.mark.parametrize("fp64_merge", ["f64i8", "f64"])
+def test_hotrg3d_chi16_bench_schedule_within_1e8_of_oracle(gpu, fp64_merge):
+    """The benchmark's schedule (its float64-step merges on the int8 MFMA by
+    default, or on the fp64 MFMA) against the oracle's exact-eigh solve."""
     from trg_backup_amd import workloads
-    z, ref = _hotrg3d(gpu, workloads.hotrg3d_policy())
+    z, ref = _hotrg3d(gpu, workloads.hotrg3d_policy(fp64_merge=fp64_merge))
     assert np.all(np.abs(z - ref) <= 1e-8)
 
 

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtrg.so")
 
 F32, F64, BF16, I32, BF16X3, BF16X6 = 0, 1, 2, 3, 4, 5
-ABI = 7          # must match the "abi N" revision in trg_version()
+ABI = 8          # must match the "abi N" revision in trg_version()
 
 c_int, c_ll, c_dbl, c_vp, c_ull, c_uint = (ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p,
                                            ctypes.c_ulonglong, ctypes.c_uint)
@@ -29,6 +29,10 @@ SIGNATURES = {
     "trg_gemm_ex": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_dbl,
                     c_vp, c_ll, c_ll, c_ll, c_vp, c_ll, c_ll, c_ll,
                     c_dbl, c_vp, c_ll, c_ll, c_ll, c_int, c_ll, c_int, c_ll, c_vp],
+    "trg_gemm_f64i8_workspace": [c_int, c_int, c_int, c_int, c_int, c_int],
+    "trg_gemm_f64i8": [c_int, c_int, c_int, c_int, c_int, c_dbl,
+                       c_vp, c_ll, c_ll, c_ll, c_vp, c_ll, c_ll, c_ll,
+                       c_dbl, c_vp, c_ll, c_ll, c_ll, c_int, c_ll, c_int, c_ll, c_vp, c_ll, c_vp],
     "trg_gemm_config_count": [c_int, c_int, c_int],
     "trg_gemm_config_name": [c_int, c_int, c_int, c_int],
     "trg_copy_strided": [c_int, c_int, c_int, P_LL, c_vp, P_LL, c_vp, P_LL, c_vp],
@@ -57,7 +61,7 @@ SIGNATURES = {
 }
 _RESTYPES = {"trg_version": ctypes.c_char_p, "trg_status_string": ctypes.c_char_p,
              "trg_gemm_config_name": ctypes.c_char_p,
-             "trg_eigh_topk_workspace": c_ll, "trg_eigh_topk_noalloc_max_bytes": c_ll, "trg_orthonormalize_workspace": c_ll}
+             "trg_eigh_topk_workspace": c_ll, "trg_gemm_f64i8_workspace": c_ll, "trg_eigh_topk_noalloc_max_bytes": c_ll, "trg_orthonormalize_workspace": c_ll}
 
 
 class TrgError(RuntimeError):

@@ -10,7 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtrg.so")
 SOURCES = ["capi.hip", "tensor_ops.hip", "gemm.hip", "gemm_bf16.hip", "gemm_x3.hip", "gemm_x6.hip", "gemm_fp.hip",
-           "linalg.hip"]
+           "gemm_i8.hip", "linalg.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", os.path.join(HERE, "..", "include")]

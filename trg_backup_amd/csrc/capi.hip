@@ -3,7 +3,7 @@
 
 // The ABI revision after "abi" changes whenever an entry point's signature does; the
 // Python binding refuses a library built with a different one.
-extern "C" const char *trg_version(void) { return "libtrg 0.7 abi 7 (gfx950)"; }
+extern "C" const char *trg_version(void) { return "libtrg 0.8 abi 8 (gfx950)"; }
 
 extern "C" const char *trg_status_string(int status) {
     switch (status) {

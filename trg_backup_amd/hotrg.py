@@ -44,9 +44,12 @@ class Precision:
     """Storage dtype of T and the GEMM compute type of the environment and
     the merge ("f64", "f32", "bf16x6" = three-piece split-bf16 products with
     the 24 bits of fp32, "bf16x3" = split-bf16 products with about 16
-    significant bits per operand, "bf16").  Eigensolvers, traces and
-    log-norms are always float64; eig_q > 0 sets the isometry eigensolver's
-    subspace iterations for the step (0: the module setting EIG / EIG3)."""
+    significant bits per operand, "bf16"; with float64 storage the merge may
+    be "f64i8": float64 products on the int8 MFMA from 6 slices of 7 bits per
+    operand, 42 bits, for the products with M, N, K >= ops.I8_MIN_K).
+    Eigensolvers, traces and log-norms are always float64; eig_q > 0 sets the
+    isometry eigensolver's subspace iterations for the step (0: the module
+    setting EIG / EIG3)."""
     store: torch.dtype = torch.float64
     env: str = "f64"
     merge: str = "f64"
@@ -56,10 +59,13 @@ class Precision:
         return None if self.store == torch.float64 else (self.env if self.env in SPLIT_COMPUTE else None)
 
     def merge_compute(self):
-        return None if self.store == torch.float64 else (self.merge if self.merge in SPLIT_COMPUTE else None)
+        if self.store == torch.float64:
+            return "f64i8" if self.merge == "f64i8" else None
+        return self.merge if self.merge in SPLIT_COMPUTE else None
 
 
 FP64 = Precision()
+FP64_I8 = Precision(torch.float64, "f64", "f64i8")
 FP32 = Precision(torch.float32, "f32", "f32")
 FP32_BF16ENV = Precision(torch.float32, "bf16", "f32")
 
@@ -70,8 +76,10 @@ class Policy:
     early steps run in float64 and later ones in lower precision."""
 
     def __init__(self, fp64_steps=10**9, low=FP32, bf16_env_from=None, bf16_merge_from=None,
-                 split_merge_from=None, split6_merge_from=None, split6_env_from=None, eig_q=()):
+                 split_merge_from=None, split6_merge_from=None, split6_env_from=None, eig_q=(), fp64_merge="f64"):
         self.fp64_steps = fp64_steps
+        # merges of the float64 steps: "f64" (fp64 MFMA) or "f64i8" (int8 MFMA slices)
+        self.fp64_merge = fp64_merge
         # ((after_step, q), ...): fewer eigensolver iterations once a step's error weighs little
         self.eig_q = tuple(eig_q or ())
         self.low = low
@@ -83,7 +91,7 @@ class Policy:
 
     def __call__(self, step):
         if step <= self.fp64_steps:
-            return FP64
+            return FP64_I8 if self.fp64_merge == "f64i8" else FP64
         env, merge = self.low.env, self.low.merge
         if self.split6_env_from is not None and step > self.split6_env_from:
             env = "bf16x6"
@@ -104,7 +112,10 @@ class Policy:
         return Precision(self.low.store, env, merge, q)
 
     def describe(self):
-        s = f"fp64 for steps <= {self.fp64_steps}, then {self.low.store} env={self.low.env} merge={self.low.merge}"
+        s = f"fp64 for steps <= {self.fp64_steps}"
+        if self.fp64_merge != "f64":
+            s += f" (merges {self.fp64_merge})"
+        s += f", then {self.low.store} env={self.low.env} merge={self.low.merge}"
         if self.split6_env_from is not None:
             s += f", bf16x6 env after step {self.split6_env_from}"
         if self.bf16_env_from is not None:

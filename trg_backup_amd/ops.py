@@ -263,6 +263,10 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_layout=N
         B = contiguous(B)
         rsB, csB, sB = B.stride(1), B.stride(2), (B.stride(0) if bt2 > 1 else 0)
     dt, cdt = dtype_code(A), dtype_code(C)
+    if compute == "f64i8" and dt == F64 and cdt == F64 and min(M, N, K) >= I8_MIN_K:
+        return _gemm_f64i8(A, B, C, alpha, beta, batch, M, N, K, sA, rsA, csA, sB, rsB, csB,
+                           Cstr if out_layout is not None else (C.stride(0) if batch > 1 else 0, C.stride(1), C.stride(2)),
+                           split, tag, batched, out_layout is not None)
     if compute is None:
         cc = BF16 if dt == BF16 else dt
     elif compute == "bf16":
@@ -273,7 +277,7 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_layout=N
         cc = BF16X6
     elif compute == "f32":
         cc = F32
-    elif compute == "f64":
+    elif compute in ("f64", "f64i8"):
         cc = F64
     else:
         cc = compute
@@ -305,6 +309,37 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, compute=None, tag=None, out_layout=N
             r.events.append((ev0, ev1, 2.0 * M * N * K * batch))
             r.event_configs.append((dt, cdt, cc, config, M, N, K, batch))
     if out_layout is not None:
+        return None
+    return C if batched else C[0]
+
+
+# float64 products on the int8 matrix cores: signed 7-bit slices per row of A
+# and column of B (6 slices: 42 bits, 21 int8 MFMAs per product); below
+# I8_MIN_K in any of M, N, K the product is bound by memory, the slicing
+# passes would add to it, and the fp64 MFMA runs
+I8_SLICES = 6
+I8_MIN_K = 256
+
+
+def _gemm_f64i8(A, B, C, alpha, beta, batch, M, N, K, sA, rsA, csA, sB, rsB, csB, Cstr, split, tag, batched,
+                layout):
+    ws = _lib.load().trg_gemm_f64i8_workspace(I8_SLICES, batch if sA else 1, batch if sB else 1, M, N, K)
+    if ws < 0:
+        raise _lib.TrgError("trg_gemm_f64i8_workspace rejected the shape")
+    work = torch.empty(max(ws, 1), dtype=torch.uint8, device=A.device)
+    recs = [r for r in _RECORDERS if tag is not None and r.tag == tag]
+    if recs:
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+    call("trg_gemm_f64i8", I8_SLICES, batch, M, N, K, float(alpha), ptr(A), sA, rsA, csA, ptr(B), sB, rsB, csB,
+         float(beta), ptr(C), *Cstr, *split, ptr(work), ws, stream_ptr())
+    if recs:
+        ev1.record()
+        for r in recs:
+            r.events.append((ev0, ev1, 2.0 * M * N * K * batch))
+            r.event_configs.append((F64, F64, "f64i8", I8_SLICES, M, N, K, batch))
+    del work   # the caching allocator keeps the block until the stream has used it
+    if layout:
         return None
     return C if batched else C[0]
 

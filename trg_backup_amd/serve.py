@@ -287,8 +287,11 @@ class Server:
                         done.append(f)
                         if log:
                             res = f"lnZ/V={f.result:.12f}" if f.error is None else f"error: {f.error}"
-                            log(f"done rid={f.req.rid} {f.req.model} chi={f.req.chi} steps={f.req.steps} "
-                                f"{res} latency={f.finished_at - f.req.arrival:.3f}s")
+                            # without real time the arrival offsets are not waited for, so only
+                            # the time since the start of the run means anything
+                            t = (f"latency={f.finished_at - f.req.arrival:.3f}s" if realtime
+                                 else f"done at {f.finished_at:.3f}s")
+                            log(f"done rid={f.req.rid} {f.req.model} chi={f.req.chi} steps={f.req.steps} {res} {t}")
                     else:
                         ready.append(f)
             # oldest request first: a stream always takes the earliest admitted ready flow (with

@@ -40,7 +40,7 @@ def hotrg3d_betas(nb=128):
 
 
 def hotrg3d_policy(fp64_steps=6, merge="bf16x6", split_merge_from=16, bf16_merge_from=None,
-                   bf16_env_from=None):
+                   bf16_env_from=None, fp64_merge="f64i8"):
     """float64 through step 6, then float32 storage with three-piece split-bf16
     GEMMs (fp32 accuracy; float32 MFMA with merge="f32"), split-bf16 merges
     after `split_merge_from`, bf16 merges after `bf16_merge_from` and bf16
@@ -48,13 +48,15 @@ def hotrg3d_policy(fp64_steps=6, merge="bf16x6", split_merge_from=16, bf16_merge
     Step 4 is the first full chi^11 step; in the ordered phase of the
     128-coupling draw a float32-accurate step 6 still moves ln Z/V by 5.8e-8
     (beta = 0.2387), so the first six steps stay float64: worst error then
-    5.5e-9 over the checked couplings, the split-bf16 merges after step 16
-    included."""
+    7.9e-9 over the eight checked couplings, the split-bf16 merges after
+    step 16 included.  The merges of those float64 steps run as 6-slice int8
+    MFMA products (fp64_merge="f64i8", 42-bit operands, 1.4x faster steps;
+    worst error 8.3e-9) or on the fp64 MFMA (fp64_merge="f64")."""
     opt = lambda v: None if v is None or v < 0 else v
     split6 = fp64_steps if merge == "bf16x6" else None
     return hotrg.Policy(fp64_steps=fp64_steps, low=hotrg.FP32, split6_merge_from=split6, split6_env_from=split6,
                         split_merge_from=opt(split_merge_from), bf16_merge_from=opt(bf16_merge_from),
-                        bf16_env_from=opt(bf16_env_from))
+                        bf16_env_from=opt(bf16_env_from), fp64_merge=fp64_merge)
 
 
 def tiers(policy, nsteps, first=1):
