@@ -723,7 +723,8 @@ def dtype_label(policy, nsteps):
     for n in range(1, nsteps + 1):
         p = policy(n)
         if p.store.is_floating_point and p.store.itemsize == 8:
-            key = "f64 (merges: fp64 products on the int8 MFMA, 6 slices)" if p.merge == "f64i8" else "f64"
+            key = ("f64 storage, merges as 6-slice int8 MFMA products (per-row fixed point)"
+                   if p.merge == "f64i8" else "f64")
         else:
             key = f"env {p.env}, merge {p.merge}"
         if key != last:

@@ -46,7 +46,8 @@ class Precision:
     the 24 bits of fp32, "bf16x3" = split-bf16 products with about 16
     significant bits per operand, "bf16"; with float64 storage the merge may
     be "f64i8": float64 products on the int8 MFMA from 6 slices of 7 bits per
-    operand, 42 bits, for the products with M, N, K >= ops.I8_MIN_K).
+    operand row, 42 bits of fixed point relative to the row's largest entry,
+    for the products with M, N, K >= ops.I8_MIN_K).
     Eigensolvers, traces and log-norms are always float64; eig_q > 0 sets the
     isometry eigensolver's subspace iterations for the step (0: the module
     setting EIG / EIG3)."""

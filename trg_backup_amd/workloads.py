@@ -50,8 +50,10 @@ def hotrg3d_policy(fp64_steps=6, merge="bf16x6", split_merge_from=16, bf16_merge
     (beta = 0.2387), so the first six steps stay float64: worst error then
     7.9e-9 over the eight checked couplings, the split-bf16 merges after
     step 16 included.  The merges of those float64 steps run as 6-slice int8
-    MFMA products (fp64_merge="f64i8", 42-bit operands, 1.4x faster steps;
-    worst error 8.3e-9) or on the fp64 MFMA (fp64_merge="f64")."""
+    MFMA products (fp64_merge="f64i8": 42 bits per row in fixed point, so
+    entries far below their row's maximum keep fewer; 1.4x faster steps,
+    about 1e-9 in ln Z/V, worst validated error 8.3e-9) or on the fp64 MFMA
+    (fp64_merge="f64")."""
     opt = lambda v: None if v is None or v < 0 else v
     split6 = fp64_steps if merge == "bf16x6" else None
     return hotrg.Policy(fp64_steps=fp64_steps, low=hotrg.FP32, split6_merge_from=split6, split6_env_from=split6,
