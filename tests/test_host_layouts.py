@@ -68,6 +68,16 @@ def test_precision_policy_tiers():
     assert pol(6).eig_q == 0 and pol(12).eig_q == 0 and pol(13).eig_q == 8 and pol(23).eig_q == 4
     assert pol(7) is hotrg.FP32 and pol(23).merge == "bf16"
     assert "4 eigensolver iterations after step 22" in pol.describe()
+    # merges of the float64 steps on the int8 MFMA: float64 storage and environments, the merge
+    # compute "f64i8"; the later tiers are unchanged
+    pol = hotrg.Policy(fp64_steps=6, low=hotrg.FP32, split6_merge_from=6, fp64_merge="f64i8")
+    assert pol(6) is hotrg.FP64_I8 and pol(6).store == torch.float64
+    assert pol(6).merge_compute() == "f64i8" and pol(6).env_compute() is None
+    assert hotrg.FP64.merge_compute() is None and pol(7).merge_compute() == "bf16x6"
+    assert "merges f64i8" in pol.describe()
+    from trg_backup_amd import workloads
+    assert workloads.hotrg3d_policy()(4).merge == "f64i8"
+    assert workloads.hotrg3d_policy(fp64_merge="f64")(4) is hotrg.FP64
 
 
 class _CpuOps:
