@@ -1097,7 +1097,7 @@ static int launch_v(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
 
 // 16-byte vector loads are legal when every vector starts on a 16-byte
 // boundary and never straddles the operand's edge.
-static bool vec_ok(const GemmArgs &g, bool amc, bool bnc, int esize, int vec) {
+static inline bool vec_ok(const GemmArgs &g, bool amc, bool bnc, int esize, int vec) {
     auto al = [&](const void *p, long long sb) {
         return (reinterpret_cast<uintptr_t>(p) % (esize * vec) == 0) && (sb % vec == 0);
     };
@@ -1114,7 +1114,7 @@ static int launch_t(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s) {
 }
 
 // Both operands K-contiguous with 16-byte vectors that never straddle K.
-static bool kvec_ok(const GemmArgs &g, bool amc, bool bnc) {
+static inline bool kvec_ok(const GemmArgs &g, bool amc, bool bnc) {
     if (amc || bnc) return false;
     auto al = [](const void *p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
     return al(g.A) && al(g.B) && g.K % 4 == 0 && g.rsA % 4 == 0 && g.csB % 4 == 0 && g.sAb % 4 == 0 &&
@@ -1157,7 +1157,7 @@ static int launch_bf16(GemmArgs g, int batch, bool amc, bool bnc, hipStream_t s)
 
 // bf16 operands in memory: both must be K-contiguous with 16-byte runs of 8
 // elements that never straddle K.
-static bool kvec8_ok(const GemmArgs &g, bool amc, bool bnc) {
+static inline bool kvec8_ok(const GemmArgs &g, bool amc, bool bnc) {
     if (amc || bnc) return false;
     auto al = [](const void *p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
     return al(g.A) && al(g.B) && g.K % 8 == 0 && g.rsA % 8 == 0 && g.csB % 8 == 0 && g.sAb % 8 == 0 &&
