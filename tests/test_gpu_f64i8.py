@@ -192,8 +192,8 @@ def test_device_k_beyond_one_chunk(gpu, any_size):
 @pytest.mark.gpu
 def test_hotrg3d_int8_merges_match_fp64_merges(gpu):
     """3D Ising chi = 8, nine steps: every float64 merge on the int8 MFMA
-    against the fp64 MFMA (the CPU study of this scheme at chi = 8 put the
-    6-slice merges of steps 4-6 within 5e-12 of float64)."""
+    against the fp64 MFMA (the oracle's chi = 8 solve with this product in
+    the merges of steps 4-6, run in NumPy, lands within 5e-12 of float64)."""
     from oracle import models
     from trg_backup_amd import hotrg
     T0 = _dev(models.ising3d_tensor(0.2387), gpu).unsqueeze(0)

@@ -1,5 +1,9 @@
 // Float64 GEMM on the int8 matrix cores (v_mfma_i32_32x32x32_i8, 5 POP/s
-// dense on MI355X against 78.6 TF/s for the fp64 MFMA).
+// dense on MI355X against 78.6 TF/s for the fp64 MFMA).  The method is the
+// error-free splitting of Ozaki, Ogita, Oishi and Rump (Numer. Algorithms 59,
+// 2012) in its integer form (Ootomo, Ozaki and Yokota, "DGEMM on integer
+// matrix multiplication unit", IJHPCA 2024): fixed-point slices per row,
+// exact integer products, the slice pairs grouped by weight.
 //
 // Each row of A (and each column of B) is written as 2^e times a value in
 // (-1, 1), and that value as S signed 7-bit slices: r_0 = x 2^-e,
