@@ -42,6 +42,32 @@ def test_bad_arguments_are_rejected_without_a_gpu():
     assert lib.trg_eigh_topk_f64(1, 10, 5, 20, 2, None, 0, None, None, None, 0, None, 0, None, None) == 2
 
 
+def test_int8_gemm_workspace_and_argument_checks_without_a_gpu():
+    """trg_gemm_f64i8's workspace: the int8 slices of A and B at a row pitch of
+    K rounded up to 64 (one K chunk: slices 127^2 K < 2^31), then the int32 row
+    exponents, each part 256-byte aligned; bad arguments return before any launch."""
+    import ctypes
+    from trg_backup_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libtrg.so not built")
+    lib = _lib.load()
+    al = lambda v: (v + 255) // 256 * 256
+    ws = lambda S, ba, bb, M, N, kp: al(ba * S * M * kp) + al(bb * S * N * kp) + al(ba * M * 4) + al(bb * N * 4)
+    assert lib.trg_gemm_f64i8_workspace(6, 1, 1, 64, 64, 64) == ws(6, 1, 1, 64, 64, 64)
+    assert lib.trg_gemm_f64i8_workspace(6, 3, 1, 100, 37, 65) == ws(6, 3, 1, 100, 37, 128)
+    chunk = (2 ** 31 - 1) // (6 * 127 * 127) // 64 * 64
+    assert lib.trg_gemm_f64i8_workspace(6, 1, 1, 8, 8, 10 ** 6) == ws(6, 1, 1, 8, 8, chunk)
+    assert lib.trg_gemm_f64i8_workspace(7, 1, 1, 0, 0, 0) == 0
+    assert lib.trg_gemm_f64i8_workspace(3, 1, 1, 4, 4, 4) == -1 and lib.trg_gemm_f64i8_workspace(8, 1, 1, 4, 4, 4) == -1
+    p = ctypes.c_void_p(4096)   # never dereferenced: every call below returns before a launch
+    args = lambda S, ws_bytes: (S, 1, 64, 64, 64, 1.0, p, 0, 64, 1, p, 0, 64, 1, 0.0, p, 0, 64, 1, 0, 0, 0, 0,
+                                p, ws_bytes, None)
+    assert lib.trg_gemm_f64i8(*args(9, 1 << 20)) == 1
+    assert lib.trg_gemm_f64i8(*args(6, 1000)) == 3
+    assert lib.trg_gemm_f64i8(6, 1, -1, 4, 4, 1.0, p, 0, 4, 1, p, 0, 4, 1, 0.0, p, 0, 4, 1, 0, 0, 0, 0, p, 1 << 20,
+                              None) == 1
+
+
 def test_library_shares_torchs_hip_runtime_whatever_the_import_order():
     """Loading the library before anything imported torch must still leave a
     single HIP runtime in the process (torch's): the library takes torch's
